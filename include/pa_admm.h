@@ -1,0 +1,287 @@
+/*
+ * pa_admm.h - C ABI of the MI355X (gfx950) ADMM frictional-contact backend.
+ *
+ * A drop-in for the GPU path of OkifuZ/parallel_admm:
+ *   pa_backend_*   replaces ADMMBackendGPU behind ADU::IADMMBackend
+ *                  (src/solver/core/admm_backend.h:15-41; implementation
+ *                  src/solver/backend_gpu/admm_impl_gpu.cu)
+ *   pa_detector_*  replaces BVH_GPU behind ADU::ICollisionDetector
+ *                  (src/contact/icollision_detector.h:19-47,
+ *                  src/contact/collision_detection_cuda.h:45-105)
+ *   pa_scene_*     host-side setup that the reference does in MeshData +
+ *                  Mesh2Constraint + the constraint constructors
+ *                  (src/mesh/mesh_container.h:232-472,
+ *                  src/constraint/mesh_to_constraint.h:29-153)
+ *
+ * Conventions: plain pointers and sizes only; every call returns pa_status
+ * (0 = success) and never exits the process (the reference prints and calls
+ * exit(), src/mutils/cuda_tools.cuh:37-46).  Positions are xyz-interleaved
+ * row-major (n x 3), y-up, gravity along -y, as in the reference.  Host
+ * buffers belong to the caller and are copied in or out; device memory
+ * belongs to the handle.  One handle is used from one host thread at a time;
+ * distinct handles are independent (the reference allows one backend per
+ * process because of function-local statics, admm_impl_gpu.cu:69-82).
+ */
+#ifndef PA_ADMM_H
+#define PA_ADMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t pa_status;
+enum {
+    PA_OK = 0,
+    PA_ERR_INVALID = 1,   /* bad argument or call order */
+    PA_ERR_HIP = 2,       /* HIP runtime error */
+    PA_ERR_NO_DEVICE = 3, /* no GPU visible */
+    PA_ERR_OVERFLOW = 4,  /* contact capacity exceeded (contacts were not truncated silently) */
+    PA_ERR_NOT_SPD = 5,   /* system matrix not positive definite */
+    PA_ERR_GEOMETRY = 6   /* degenerate or inverted rest geometry */
+};
+
+/* ---------------------------------------------------------------------------
+ * Solver configuration.  Mirrors ADU::ADMMSolverConfig
+ * (src/solver/core/admm_config.h:13-55) and the scene keys that feed it
+ * (src/solver/solver_config_applier.cpp:5-53), plus the convergence controls
+ * this backend adds (the reference computes no residual).
+ * ------------------------------------------------------------------------- */
+typedef struct pa_admm_config {
+    double dt;              /* time step h */
+    double g;               /* gravity magnitude along -y */
+    int32_t admm_max_iter;  /* admm_max_iter */
+    int32_t dcd_interval;   /* DCD_interval: detection every k ADMM iterations */
+    int32_t gs_max_iter;    /* GS_max_iter: contact projection sweeps per iteration */
+    int32_t warmstart_ue;   /* admm.warmstart_Ue */
+    int32_t warmstart_uc;   /* admm.warmstart_Uc */
+    int32_t enable_contact; /* contact.enable */
+    double mu;              /* contact.mu */
+    double kappa;           /* contact.kappa */
+    double beta;            /* contact.beta */
+    double thickness;       /* narrow.thickness (d-hat) */
+    double damp_kl;         /* damp.kl */
+    double damp_km;         /* damp.km */
+    double w_scale;         /* contact.w_scale: w_c = m / h^2 * w_scale */
+    double residual_tol;    /* stop when max(r_primal, r_dual) <= tol; 0 = run admm_max_iter */
+    int32_t residual_interval; /* check the residual every k iterations */
+    int32_t global_max_iter;   /* cap on PCG iterations per refinement pass (Global_Jacobi_iter) */
+    double global_tol;         /* PCG stop per pass: |r_k| <= global_tol |r_0|, r_0 = b - A x in FP64 */
+    int32_t max_contacts;      /* narrow.max_collision: contact capacity */
+    int32_t global_refine;     /* refinement passes per global solve (FP64 residual, FP32 PCG correction) */
+} pa_admm_config;
+
+/* Fills the reference's ADMMSolverConfig defaults (admm_config.h:20-53). */
+void pa_admm_config_default(pa_admm_config* cfg);
+
+/* ---------------------------------------------------------------------------
+ * Scene: mesh + topology + elastic constraints + lumped mass.
+ * Vertex order: dynamic vertices [0, n_dyn) then static obstacle vertices
+ * [n_dyn, n_all) (static_vert_begin, src/config/mesh_from_config.h:63-82).
+ * Constraint rows: triangles (2 rows), tets (3), bending (1), pins (1), in
+ * that order (mesh_to_constraint.h + app_initializer.cpp:145-172).
+ * ------------------------------------------------------------------------- */
+typedef struct pa_scene pa_scene;
+
+typedef struct pa_mesh_desc {
+    int32_t n_all, n_dyn;
+    const double* x;           /* 3 n_all */
+    const double* density;     /* n_all (static vertices: 1e12) */
+    int32_t n_tri;
+    const int32_t* tris;       /* 3 n_tri: dynamic cloth faces first, then static faces */
+    const double* tri_k;       /* n_tri stretch stiffness (ignored for static faces) */
+    const double* bend_k;      /* n_all bending stiffness per vertex */
+    int32_t n_tet;
+    const int32_t* tets;       /* 4 n_tet */
+    const double* tet_k;       /* n_tet */
+    int32_t n_tet_boundary;
+    const int32_t* tet_boundary; /* 3 n_tet_boundary: surface facets of the tet meshes */
+    int32_t n_pins;
+    const int32_t* pins;
+} pa_mesh_desc;
+
+/* Build topology, constraints and lumped mass from a raw mesh
+ * (MeshData::precompute_geometry_info + Mesh2Constraint + get_mass). */
+pa_status pa_scene_create(const pa_mesh_desc* mesh, pa_scene** out);
+
+/* Generators for the benchmark scenes (parameters in the implementation):
+ *   kind 1: 32x32 cloth (1 cm) over a level-3 icosphere (r 0.2 m), two pinned corners
+ *   kind 2: 20x20x20-cell 5-tet box (0.2 m) with a seeded random rotation over a ground quad
+ *   kind 3: 256x256 cloth (2 mm) over a sphere (r 0.15 m) and a cylinder (r 0.05 m)
+ *   kind 4: one cloth of side `side` (16..96) over a sphere of radius `radius` at drop height `drop`
+ * `side`, `seed`, `radius` and `drop` are used where the kind needs them (0 = default). */
+pa_status pa_scene_generate(int32_t kind, int32_t side, uint64_t seed, double radius, double drop, pa_scene** out);
+void pa_scene_destroy(pa_scene* s);
+
+typedef struct pa_scene_info {
+    int32_t n_all, n_dyn, m_rows;
+    int32_t n_tri, n_tet, n_tet_boundary, n_pins;
+    int32_t n_tri_c, n_tet_c, n_bend_c, n_pin_c; /* constraints per type */
+    int32_t n_surf_verts, n_surf_tris, n_surf_edges;
+    int64_t nnz_d, bend_entries;
+} pa_scene_info;
+pa_status pa_scene_get_info(const pa_scene* s, pa_scene_info* out);
+
+/* Copies of the raw mesh (arrays sized as in pa_scene_info; NULL skips). */
+pa_status pa_scene_get_mesh(const pa_scene* s, double* x, double* density, int32_t* tris, double* tri_k, double* bend_k,
+                            int32_t* tets, double* tet_k, int32_t* tet_boundary, int32_t* pins);
+/* Surface topology: sorted surface vertices, surface triangles, surface edges (min,max). */
+pa_status pa_scene_get_topology(const pa_scene* s, int32_t* surf_verts, int32_t* surf_tris, int32_t* surf_edges);
+pa_status pa_scene_get_mass(const pa_scene* s, double* mass);
+
+/* Flattened constraints (≙ the Constraint objects passed to add_constraints,
+ * solver.h:96; tri/tet Binv row-major, bending ring + coefficients). */
+typedef struct pa_constraint_soa {
+    int32_t n_tri, n_tet, n_bend, n_pin;
+    int32_t tri_row0, tet_row0, bend_row0, pin_row0; /* first row of each block; rows contiguous within a type */
+    const int32_t* tri_inds;  /* 3 n_tri */
+    const double* tri_binv;   /* 4 n_tri */
+    const double* tri_w;      /* n_tri */
+    const int32_t* tet_inds;  /* 4 n_tet */
+    const double* tet_binv;   /* 9 n_tet */
+    const double* tet_w;      /* n_tet */
+    const int32_t* bend_ptr;  /* n_bend + 1 */
+    const int32_t* bend_inds; /* middle vertex then ring, bend_ptr[n_bend] entries */
+    const double* bend_coef;  /* one per bend_inds entry */
+    const double* bend_w;     /* n_bend */
+    const int32_t* pin_inds;  /* n_pin */
+    const double* pin_pos;    /* 3 n_pin */
+    const double* pin_w;      /* n_pin */
+} pa_constraint_soa;
+
+/* Writable views into the scene's own constraint arrays (valid while the scene lives). */
+pa_status pa_scene_get_constraints(const pa_scene* s, pa_constraint_soa* out);
+
+/* ---------------------------------------------------------------------------
+ * Collision detector (≙ ICollisionDetector / BVH_GPU): LBVH over the surface
+ * triangles and edges, canonical sorted contact list.
+ * ------------------------------------------------------------------------- */
+typedef struct pa_detector pa_detector;
+
+typedef struct pa_surface {
+    int32_t n_all, n_dyn;
+    int32_t n_surf_verts, n_surf_tris, n_surf_edges;
+    const int32_t* surf_verts; /* sorted */
+    const int32_t* surf_tris;  /* 3 n_surf_tris */
+    const int32_t* surf_edges; /* 2 n_surf_edges, (min, max) */
+} pa_surface;
+
+/* ≙ BVH_GPU::init(max_collision) (collision_detection_cuda.cpp:39-83). */
+pa_status pa_detector_create(const pa_surface* surf, int32_t capacity, double thickness, pa_detector** out);
+void pa_detector_destroy(pa_detector* d);
+/* ≙ ICollisionDetector::detect(pos): positions from the host (3 n_all doubles,
+ * rounded to FP32 on upload). */
+pa_status pa_detector_detect(pa_detector* d, const double* x_host, int32_t* n_contacts);
+/* Canonical contact list of the last detection: ints = 7 per contact
+ * (type 0=PT/1=EE, prim0, prim1, v0..v3); reals = 9 per contact (bary[4],
+ * normal[3], h_cN, distance).  Order: PT by (vertex, face), then EE by
+ * (edge i, edge j), i < j. */
+pa_status pa_detector_get_contacts(pa_detector* d, int32_t* ints, double* reals, int32_t cap, int32_t* n_contacts);
+/* contact_count / max_slots / peak_contact_count (icollision_detector.h:30-36). */
+pa_status pa_detector_counts(const pa_detector* d, int32_t* count, int32_t* max_slots, int32_t* peak);
+/* points() / normals() (icollision_detector.h:38-44): exactly max_slots
+ * entries each, unused slots zero-filled. */
+pa_status pa_detector_points(pa_detector* d, float* points3);
+pa_status pa_detector_normals(pa_detector* d, float* normals3);
+/* clear() */
+pa_status pa_detector_clear(pa_detector* d);
+
+/* ---------------------------------------------------------------------------
+ * Backend (≙ IADMMBackend + the Solver::init/reset/getVertices entry points
+ * reached through as_solver(), core/admm_solver.cpp:5-17, solver.h:66-137).
+ * Call order (app_initializer.cpp:109-224): create -> apply_config ->
+ * set_constraints -> set_constraint_dim -> finalize_constraints -> init ->
+ * set_sync_to_host -> attach_detector; then step() per sub-step.
+ * ------------------------------------------------------------------------- */
+typedef struct pa_backend pa_backend;
+
+typedef struct pa_device_opts {
+    int32_t device;  /* HIP device ordinal */
+    int32_t verbose; /* 0: silent */
+} pa_device_opts;
+
+typedef struct pa_step_stats {
+    int32_t iterations;    /* ADMM iterations run */
+    int32_t n_contacts;    /* contacts of the last detection */
+    int32_t converged;     /* residual reached residual_tol */
+    int32_t overflow;      /* contact capacity exceeded */
+    int32_t global_iters;  /* PCG iterations summed over the step */
+    int32_t detections;
+    double r_primal, r_dual; /* last residual check */
+    double step_ms;          /* host wall time of the step */
+} pa_step_stats;
+
+pa_status pa_backend_create(const pa_device_opts* opts, pa_backend** out);
+void pa_backend_destroy(pa_backend* b);
+const char* pa_backend_last_error(const pa_backend* b);
+
+/* ≙ IADMMBackend::apply_config (before init). */
+pa_status pa_backend_apply_config(pa_backend* b, const pa_admm_config* cfg);
+/* ≙ Solver::add_constraints + addPins: constraint data (copied). */
+pa_status pa_backend_set_constraints(pa_backend* b, const pa_constraint_soa* c);
+/* ≙ IADMMBackend::set_constraint_dim(m_nCDim). */
+pa_status pa_backend_set_constraint_dim(pa_backend* b, int32_t m_rows);
+/* ≙ IADMMBackend::finalize_constraints: checks rows, packs device SoA. */
+pa_status pa_backend_finalize_constraints(pa_backend* b);
+/* ≙ Solver::init(verts, mass, dt) + ADMMImplGPU::init/precompute: assembles
+ * M, D, We, Damp, Wc and A, uploads them and the state.  surface_flags marks
+ * surface vertices (vinds_surf_set), pins lists the pinned vertices. */
+pa_status pa_backend_init(pa_backend* b, const double* x0, const double* mass, int32_t n_all, int32_t n_dyn,
+                          const uint8_t* surface_flags, int32_t n_pins, const int32_t* pins);
+/* ≙ set_sync_to_host: copy x, v back to host after each step. */
+pa_status pa_backend_set_sync_to_host(pa_backend* b, int32_t enabled);
+/* ≙ Solver::bvh pointer: the detector runs on the backend's device positions
+ * and stream.  The detector must outlive the backend's use of it. */
+pa_status pa_backend_attach_detector(pa_backend* b, pa_detector* d);
+/* ≙ IADMMBackend::step. */
+pa_status pa_backend_step(pa_backend* b, pa_step_stats* out);
+/* ≙ Solver::reset(ini_verts): state <- x0, v <- 0, duals <- 0. */
+pa_status pa_backend_reset(pa_backend* b, const double* x0);
+/* ≙ getVertices / getVelocities (explicit copy-out, 3 n_all each; NULL skips). */
+pa_status pa_backend_get_state(pa_backend* b, double* x, double* v);
+pa_status pa_backend_set_state(pa_backend* b, const double* x, const double* v);
+/* Dual variables Ue (3 m) and Uc (3 n_dyn). */
+pa_status pa_backend_get_duals(pa_backend* b, double* ue, double* uc);
+pa_status pa_backend_set_duals(pa_backend* b, const double* ue, const double* uc);
+/* Static obstacle motion for the next step (≙ ScriptAnimator::animate_all,
+ * src/solver/animator.cpp:199-205): positions and velocities of [n_dyn, n_all). */
+pa_status pa_backend_set_static_state(pa_backend* b, const double* x_static, const double* v_static);
+
+/* Convenience: runs the whole call order above for a scene (creates and
+ * attaches its own detector when contact is enabled). */
+pa_status pa_backend_create_from_scene(const pa_scene* s, const pa_admm_config* cfg, const pa_device_opts* opts,
+                                       pa_backend** out);
+
+/* ---------------------------------------------------------------------------
+ * Sub-step entry points for parity tests and profiling.  Each runs one
+ * device phase on the backend's current device state.
+ * ------------------------------------------------------------------------- */
+/* Local step: z = prox(D x + Ue) for host x (3 n_all) and Ue (3 m); writes z (3 m). */
+pa_status pa_backend_project_elastic(pa_backend* b, const double* x, const double* ue, double* z);
+/* Contact projection with the detector's current contacts: p (3 n_all) in/out,
+ * r_c (3 n_contacts) out.  Runs the Scc setup and gs_max_iter Jacobi sweeps. */
+pa_status pa_backend_project_contacts(pa_backend* b, double* p, double* rc);
+/* Global solve A x = rhs (3 n_dyn) to global_tol, from x_init (3 n_dyn); returns PCG iterations. */
+pa_status pa_backend_global_solve(pa_backend* b, const double* rhs, const double* x_init, double* x, int32_t* iters);
+/* hipEvent timing (adds event records to the stream; off by default):
+ * mode bit 0 times every PCG SpMV launch, bit 1 times the step phases. */
+pa_status pa_backend_set_profiling(pa_backend* b, int32_t mode);
+/* Device timing of the last step by phase (ms): detection, local, contact, rhs, global, other. */
+pa_status pa_backend_phase_times(pa_backend* b, double* ms6);
+/* Sizes: [n_all, n_dyn, m_rows, nnz of the off-diagonal system matrix,
+ * nnz of the damping matrix, nnz of the D^T gather, contacts of the last detection]. */
+pa_status pa_backend_get_sizes(pa_backend* b, int64_t* out7);
+/* Per-launch average duration (ms, HIP events on the backend's stream) and
+ * launch count of the PCG SpMV kernel over the timed steps since the last call
+ * with reset = 1. */
+pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, double* avg_ms, int64_t* launches);
+
+/* Message of the last failed call on this host thread (any handle). */
+const char* pa_last_error_message(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PA_ADMM_H */

@@ -1,0 +1,1512 @@
+// FP64 CPU oracle for the ADMM contact step (TEST INFRASTRUCTURE ONLY).
+//
+// This is a restatement, written for this project, of the algorithm that the
+// reference's CPU backend runs (src/solver/backend_cpu/admm_impl_cpu.cpp) with
+// the contact sub-solver of its GPU backend (src/solver/contact_solver.cu),
+// templated on the scalar type: R = double is the oracle, R = float is the
+// reference-faithful twin (the reference's ADU::Real is float,
+// src/mutils/common_types.h:10).  Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may call it; the product library never links it.
+//
+// Pinning (which independent check fixes each part):
+//   dist_pt / dist_ee        closed forms + brute-force (s,t) grid minimum (tests/test_oracle_geometry.py)
+//   detect (contact set)     O(N^2) loop with no broad phase == broad-phase result; invariants
+//   tri/tet prox             polar-decomposition invariants, closed forms F = R*diag, F = I
+//   bend / pin prox          closed forms (z/2, pin position)
+//   sparse LLT               ||A x - b|| / ||b|| <= 1e-12 against a dense reference (numpy)
+//   topology + constraints   counts quoted for the configs, D*X_rest identities (F = I at rest)
+//   contact projection       closed-form cone cases, one-contact fixed point
+//   whole step               free-fall closed form, pinned vertices fixed, g = 0 rest state
+#pragma once
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+namespace orc {
+
+// ---------------------------------------------------------------------------
+// 3-vectors.  dot() sums ((x + y) + z); the product kernels use the same order,
+// and every file here is compiled with -ffp-contract=off.
+// ---------------------------------------------------------------------------
+template <class R> struct V3 {
+    R x, y, z;
+};
+template <class R> inline V3<R> mk(R x, R y, R z) { return V3<R>{x, y, z}; }
+template <class R> inline V3<R> operator+(V3<R> a, V3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class R> inline V3<R> operator-(V3<R> a, V3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class R> inline V3<R> operator*(R s, V3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <class R> inline R dot(V3<R> a, V3<R> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class R> inline V3<R> cross(V3<R> a, V3<R> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class R> inline R norm(V3<R> a) { return std::sqrt(dot(a, a)); }
+// Eigen's normalized(): v / sqrt(|v|^2) when |v|^2 > 0, else v unchanged.
+template <class R> inline V3<R> normalized(V3<R> a) {
+    R n2 = dot(a, a);
+    if (n2 > R(0)) {
+        R n = std::sqrt(n2);
+        return {a.x / n, a.y / n, a.z / n};
+    }
+    return a;
+}
+template <class R> inline V3<R> load3(const R* p, int64_t i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+template <class R> inline void store3(R* p, int64_t i, V3<R> v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}
+
+// ---------------------------------------------------------------------------
+// Point-triangle distance: Eberly, "Distance Between Point and Triangle in 3D"
+// (Geometric Tools, DistancePoint3Triangle3.pdf), quadratic minimisation over the
+// triangle's parameter domain split into regions 0..6.  Same region logic as the
+// reference's src/mutils/distPT.h:59-267.
+// ---------------------------------------------------------------------------
+template <class R> struct PTResult {
+    R distance, sqr_distance;
+    R bary[3];        // closest = bary0*t0 + bary1*t1 + bary2*t2
+    V3<R> closest[2]; // [0] = point, [1] = closest point on triangle
+};
+
+template <class R> PTResult<R> dist_pt(V3<R> p, V3<R> t0, V3<R> t1, V3<R> t2) {
+    const R zero = 0, one = 1, two = 2;
+    V3<R> diff = t0 - p;
+    V3<R> e0 = t1 - t0;
+    V3<R> e1 = t2 - t0;
+    R a00 = dot(e0, e0), a01 = dot(e0, e1), a11 = dot(e1, e1);
+    R b0 = dot(diff, e0), b1 = dot(diff, e1);
+    R det = std::max(a00 * a11 - a01 * a01, zero);
+    R s = a01 * b1 - a11 * b0;
+    R t = a01 * b0 - a00 * b1;
+    if (s + t <= det) {
+        if (s < zero) {
+            if (t < zero) { // region 4
+                if (b0 < zero) {
+                    t = zero;
+                    s = (-b0 >= a00) ? one : -b0 / a00;
+                } else {
+                    s = zero;
+                    if (b1 >= zero) t = zero;
+                    else if (-b1 >= a11) t = one;
+                    else t = -b1 / a11;
+                }
+            } else { // region 3
+                s = zero;
+                if (b1 >= zero) t = zero;
+                else if (-b1 >= a11) t = one;
+                else t = -b1 / a11;
+            }
+        } else if (t < zero) { // region 5
+            t = zero;
+            if (b0 >= zero) s = zero;
+            else if (-b0 >= a00) s = one;
+            else s = -b0 / a00;
+        } else { // region 0: interior
+            s /= det;
+            t /= det;
+        }
+    } else {
+        if (s < zero) { // region 2
+            R tmp0 = a01 + b0, tmp1 = a11 + b1;
+            if (tmp1 > tmp0) {
+                R numer = tmp1 - tmp0;
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { s = one; t = zero; }
+                else { s = numer / denom; t = one - s; }
+            } else {
+                s = zero;
+                if (tmp1 <= zero) t = one;
+                else if (b1 >= zero) t = zero;
+                else t = -b1 / a11;
+            }
+        } else if (t < zero) { // region 6
+            R tmp0 = a01 + b1, tmp1 = a00 + b0;
+            if (tmp1 > tmp0) {
+                R numer = tmp1 - tmp0;
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { t = one; s = zero; }
+                else { t = numer / denom; s = one - t; }
+            } else {
+                t = zero;
+                if (tmp1 <= zero) s = one;
+                else if (b0 >= zero) s = zero;
+                else s = -b0 / a00;
+            }
+        } else { // region 1
+            R numer = a11 + b1 - a01 - b0;
+            if (numer <= zero) { s = zero; t = one; }
+            else {
+                R denom = a00 - two * a01 + a11;
+                if (numer >= denom) { s = one; t = zero; }
+                else { s = numer / denom; t = one - s; }
+            }
+        }
+    }
+    PTResult<R> r;
+    r.closest[0] = p;
+    r.closest[1] = (t0 + s * e0) + t * e1;
+    V3<R> d = r.closest[0] - r.closest[1];
+    r.sqr_distance = dot(d, d);
+    r.distance = std::sqrt(r.sqr_distance);
+    r.bary[0] = (one - s) - t;
+    r.bary[1] = s;
+    r.bary[2] = t;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Segment-segment distance: Eberly, "Distance Between Two Line Segments"
+// (Geometric Tools, DistanceLine3Line3.pdf / robust segment algorithm), regions
+// 0..8 of the (s,t) unit square, with the parallel case handled separately.
+// Same region logic as the reference's src/mutils/distEE.h:62-300.
+// ---------------------------------------------------------------------------
+template <class R> struct EEResult {
+    R distance, sqr_distance;
+    R param[2];
+    V3<R> closest[2];
+};
+
+template <class R> EEResult<R> dist_ee(V3<R> P0, V3<R> P1, V3<R> Q0, V3<R> Q1) {
+    const R zero = 0, one = 1;
+    V3<R> P = P1 - P0, Q = Q1 - Q0, PQ = P0 - Q0;
+    R a = dot(P, P), b = dot(P, Q), c = dot(Q, Q), d = dot(P, PQ), e = dot(Q, PQ);
+    R det = a * c - b * b;
+    R s = zero, t = zero;
+    if (det > zero) {
+        R bte = b * e, ctd = c * d;
+        if (bte <= ctd) { // s <= 0
+            s = zero;
+            if (e <= zero) { // region 6
+                t = zero;
+                R nd = -d;
+                if (nd >= a) s = one;
+                else if (nd > zero) s = nd / a;
+            } else if (e < c) { // region 5
+                t = e / c;
+            } else { // region 4
+                t = one;
+                R bmd = b - d;
+                if (bmd >= a) s = one;
+                else if (bmd > zero) s = bmd / a;
+            }
+        } else { // s > 0
+            s = bte - ctd;
+            if (s >= det) { // s >= 1
+                s = one;
+                R bpe = b + e;
+                if (bpe <= zero) { // region 8
+                    t = zero;
+                    R nd = -d;
+                    if (nd <= zero) s = zero;
+                    else if (nd < a) s = nd / a;
+                } else if (bpe < c) { // region 1
+                    t = bpe / c;
+                } else { // region 2
+                    t = one;
+                    R bmd = b - d;
+                    if (bmd <= zero) s = zero;
+                    else if (bmd < a) s = bmd / a;
+                }
+            } else { // 0 < s < 1
+                R ate = a * e, btd = b * d;
+                if (ate <= btd) { // region 7
+                    t = zero;
+                    R nd = -d;
+                    if (nd <= zero) s = zero;
+                    else if (nd >= a) s = one;
+                    else s = nd / a;
+                } else {
+                    t = ate - btd;
+                    if (t >= det) { // region 3
+                        t = one;
+                        R bmd = b - d;
+                        if (bmd <= zero) s = zero;
+                        else if (bmd >= a) s = one;
+                        else s = bmd / a;
+                    } else { // region 0
+                        s /= det;
+                        t /= det;
+                    }
+                }
+            }
+        }
+    } else { // parallel segments
+        if (e <= zero) {
+            t = zero;
+            R nd = -d;
+            if (nd <= zero) s = zero;
+            else if (nd >= a) s = one;
+            else s = nd / a;
+        } else if (e >= c) {
+            t = one;
+            R bmd = b - d;
+            if (bmd <= zero) s = zero;
+            else if (bmd >= a) s = one;
+            else s = bmd / a;
+        } else {
+            s = zero;
+            t = e / c;
+        }
+    }
+    EEResult<R> r;
+    r.param[0] = s;
+    r.param[1] = t;
+    r.closest[0] = P0 + s * P;
+    r.closest[1] = Q0 + t * Q;
+    V3<R> dd = r.closest[0] - r.closest[1];
+    r.sqr_distance = dot(dd, dd);
+    r.distance = std::sqrt(r.sqr_distance);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Mesh topology (reference: src/mesh/mesh_container.h:232-335).
+//   surface_tris : given by the caller (tri faces, then tet boundary facets)
+//   surface_vinds: sorted unique vertices of surface_tris
+//   surface_edges: unique undirected edges (min,max) of the tri faces, ordered
+//                  by (max, min) (igl::edges order), then those of the tet
+//                  boundary facets in the same order.
+//   adjacency    : per-vertex sorted unique neighbours over tri faces
+//                  (igl::adjacency_list with sorted = false sorts + uniques).
+// ---------------------------------------------------------------------------
+inline std::vector<std::array<int, 2>> unique_edges(const std::vector<std::array<int, 3>>& tris) {
+    std::vector<std::array<int, 2>> e;
+    e.reserve(tris.size() * 3);
+    for (auto& f : tris)
+        for (int k = 0; k < 3; k++) {
+            int a = f[k], b = f[(k + 1) % 3];
+            e.push_back({std::min(a, b), std::max(a, b)});
+        }
+    std::sort(e.begin(), e.end(), [](const std::array<int, 2>& u, const std::array<int, 2>& v) {
+        return u[1] != v[1] ? u[1] < v[1] : u[0] < v[0];
+    });
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+    return e;
+}
+
+struct Topology {
+    int n_all = 0, n_dyn = 0;
+    std::vector<std::array<int, 3>> surface_tris;
+    std::vector<int> surface_vinds;
+    std::vector<std::array<int, 2>> surface_edges;
+    std::vector<std::vector<int>> adjacency;
+    std::vector<uint8_t> is_static;
+};
+
+inline Topology build_topology(int n_all, int n_dyn, const std::vector<std::array<int, 3>>& tri_faces,
+                               const std::vector<std::array<int, 3>>& tet_boundary) {
+    Topology T;
+    T.n_all = n_all;
+    T.n_dyn = n_dyn;
+    T.is_static.assign(n_all, 0);
+    for (int i = n_dyn; i < n_all; i++) T.is_static[i] = 1;
+    T.surface_tris = tri_faces;
+    T.surface_tris.insert(T.surface_tris.end(), tet_boundary.begin(), tet_boundary.end());
+    std::vector<char> seen(n_all, 0);
+    for (auto& f : T.surface_tris)
+        for (int k = 0; k < 3; k++) seen[f[k]] = 1;
+    for (int i = 0; i < n_all; i++)
+        if (seen[i]) T.surface_vinds.push_back(i);
+    T.surface_edges = unique_edges(tri_faces);
+    auto eb = unique_edges(tet_boundary);
+    T.surface_edges.insert(T.surface_edges.end(), eb.begin(), eb.end());
+    T.adjacency.assign(n_all, {});
+    for (auto& f : tri_faces)
+        for (int k = 0; k < 3; k++) {
+            int s = f[k], d = f[(k + 1) % 3];
+            T.adjacency[s].push_back(d);
+            T.adjacency[d].push_back(s);
+        }
+    for (auto& a : T.adjacency) {
+        std::sort(a.begin(), a.end());
+        a.erase(std::unique(a.begin(), a.end()), a.end());
+    }
+    return T;
+}
+
+// ---------------------------------------------------------------------------
+// Elastic constraints (reference: src/constraint/*.h, factory
+// src/constraint/mesh_to_constraint.h:29-153).  Row layout: all triangle
+// constraints (2 rows each), then tets (3 rows), then bending (1 row), then pins.
+// Each row r of D holds coefficients over vertex columns; We(r) = w.
+// ---------------------------------------------------------------------------
+enum CType { C_TRI = 1, C_TET = 2, C_BEND = 3, C_PIN = 4 };
+
+struct Constraint {
+    int type = 0;
+    int start_row = 0;
+    int dim = 0;
+    std::vector<int> inds;
+    double w = 0, k = 0;
+    double measure = 0;          // area / volume / bending vertex area
+    std::vector<double> G;       // D block, row-major (dim x |inds|)
+    std::array<double, 3> pin{}; // pin position
+};
+
+struct SceneInput {
+    int n_all = 0, n_dyn = 0;
+    std::vector<double> x;            // 3*n_all rest / initial positions
+    std::vector<double> density;      // per vertex (static: 1e12)
+    std::vector<std::array<int, 3>> tri_faces;    // all tri faces (dynamic cloth first, then static meshes)
+    std::vector<double> tri_k;        // stretch k per tri face (ignored for static faces)
+    std::vector<double> bend_k_vert;  // bending k per vertex (tri meshes)
+    std::vector<std::array<int, 4>> tets;
+    std::vector<double> tet_k;
+    std::vector<std::array<int, 3>> tet_boundary;
+    std::vector<int> pins;
+};
+
+inline void inv2(const double a[4], double o[4]) {
+    double det = a[0] * a[3] - a[1] * a[2];
+    o[0] = a[3] / det;
+    o[1] = -a[1] / det;
+    o[2] = -a[2] / det;
+    o[3] = a[0] / det;
+}
+
+inline double det3(const double m[9]) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) + m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+inline void inv3(const double m[9], double o[9]) {
+    double d = det3(m);
+    o[0] = (m[4] * m[8] - m[5] * m[7]) / d;
+    o[1] = (m[2] * m[7] - m[1] * m[8]) / d;
+    o[2] = (m[1] * m[5] - m[2] * m[4]) / d;
+    o[3] = (m[5] * m[6] - m[3] * m[8]) / d;
+    o[4] = (m[0] * m[8] - m[2] * m[6]) / d;
+    o[5] = (m[2] * m[3] - m[0] * m[5]) / d;
+    o[6] = (m[3] * m[7] - m[4] * m[6]) / d;
+    o[7] = (m[1] * m[6] - m[0] * m[7]) / d;
+    o[8] = (m[0] * m[4] - m[1] * m[3]) / d;
+}
+
+// Triangle: reference src/constraint/triangle_constraint.h:66-137.
+// Binv = (basis^T [e12 e13])^{-1}, basis = (n1, n2) Gram-Schmidt of (e12, e13);
+// area = det/2; w = sqrt(k area); D rows: G = S * Binv with S = [[-1,-1],[1,0],[0,1]].
+inline Constraint make_tri(const std::array<int, 3>& f, const std::vector<double>& x, double k, int row) {
+    Constraint c;
+    c.type = C_TRI;
+    c.start_row = row;
+    c.dim = 2;
+    c.k = k;
+    c.inds = {f[0], f[1], f[2]};
+    V3<double> x0 = load3(x.data(), f[0]), x1 = load3(x.data(), f[1]), x2 = load3(x.data(), f[2]);
+    V3<double> e12 = x1 - x0, e13 = x2 - x0;
+    V3<double> n1 = normalized(e12);
+    V3<double> n2 = normalized(e13 - dot(e13, n1) * n1);
+    double B[4] = {dot(n1, e12), dot(n1, e13), dot(n2, e12), dot(n2, e13)};
+    double det = B[0] * B[3] - B[1] * B[2];
+    double Bi[4];
+    inv2(B, Bi);
+    c.measure = det / 2.0;
+    if (c.measure < 0) throw std::runtime_error("triangle constraint: inverted rest pose");
+    c.w = std::sqrt(k * c.measure);
+    // G (3x2) = S (3x2) * Binv (2x2); D row r over vertex j is G(j, r).
+    double S[6] = {-1, -1, 1, 0, 0, 1};
+    c.G.assign(6, 0.0);
+    for (int j = 0; j < 3; j++)
+        for (int r = 0; r < 2; r++) c.G[r * 3 + j] = S[j * 2 + 0] * Bi[0 * 2 + r] + S[j * 2 + 1] * Bi[1 * 2 + r];
+    return c;
+}
+
+// Tet: reference src/constraint/tetrahedral_constraint.h:22-89.
+inline Constraint make_tet(const std::array<int, 4>& t, const std::vector<double>& x, double k, int row) {
+    Constraint c;
+    c.type = C_TET;
+    c.start_row = row;
+    c.dim = 3;
+    c.k = k;
+    c.inds = {t[0], t[1], t[2], t[3]};
+    V3<double> x0 = load3(x.data(), t[0]);
+    V3<double> e[3] = {load3(x.data(), t[1]) - x0, load3(x.data(), t[2]) - x0, load3(x.data(), t[3]) - x0};
+    double B[9];
+    for (int r = 0; r < 3; r++)
+        for (int col = 0; col < 3; col++) {
+            const V3<double>& v = e[col];
+            B[r * 3 + col] = r == 0 ? v.x : (r == 1 ? v.y : v.z);
+        }
+    double Bi[9];
+    inv3(B, Bi);
+    c.measure = std::fabs(det3(B) / 6.0);
+    c.w = std::sqrt(k * c.measure);
+    double S[12] = {-1, -1, -1, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+    c.G.assign(12, 0.0);
+    for (int j = 0; j < 4; j++)
+        for (int r = 0; r < 3; r++) {
+            double s = 0;
+            for (int q = 0; q < 3; q++) s += S[j * 3 + q] * Bi[q * 3 + r];
+            c.G[r * 4 + j] = s;
+        }
+    return c;
+}
+
+// Bending: reference src/constraint/bending_constraint.h:24-193.  Ring order is
+// the sorted neighbour list; coefficient i sums cot(angle at middle vertex)
+// towards ring[(i+1)%n] and ring[(i-1+n)%n] unless collinear (|cross|^2 < 1e-7),
+// scaled by 1/(2 varea)/100; varea = 3.1415926 r^2 / 3 with r the mean ring
+// distance; middle coefficient = sum, ring coefficients negated.
+inline Constraint make_bend(int mid, const std::vector<int>& ring, const std::vector<double>& x, double k, int row) {
+    Constraint c;
+    c.type = C_BEND;
+    c.start_row = row;
+    c.dim = 1;
+    c.k = k;
+    c.inds.push_back(mid);
+    c.inds.insert(c.inds.end(), ring.begin(), ring.end());
+    V3<double> vm = load3(x.data(), mid);
+    int n = (int)ring.size();
+    double radius = 0;
+    for (int i = 0; i < n; i++) radius += norm(load3(x.data(), ring[i]) - vm);
+    radius /= n;
+    double varea = 3.1415926 * radius * radius / 3.0;
+    auto colinear = [](V3<double> a, V3<double> b, V3<double> cc) {
+        V3<double> cr = cross(a - b, cc - b);
+        return dot(cr, cr) < 1e-7;
+    };
+    auto cotv = [](V3<double> a1, V3<double> b, V3<double> cc) {
+        V3<double> a = a1 - b, bb = cc - b;
+        return dot(a, bb) / norm(cross(a, bb));
+    };
+    std::vector<double> co(n);
+    for (int i = 0; i < n; i++) {
+        V3<double> v = load3(x.data(), ring[i]);
+        V3<double> vcw = load3(x.data(), ring[(i + 1) % n]);
+        V3<double> vccw = load3(x.data(), ring[(i + n - 1) % n]);
+        double fv = 0;
+        if (!colinear(v, vm, vcw)) fv += cotv(v, vm, vcw);
+        if (!colinear(v, vm, vccw)) fv += cotv(v, vm, vccw);
+        co[i] = fv / (2 * varea) / 100.0;
+    }
+    double midc = 0;
+    for (int i = 0; i < n; i++) {
+        midc += co[i];
+        co[i] = -co[i];
+    }
+    c.G.push_back(midc);
+    c.G.insert(c.G.end(), co.begin(), co.end());
+    c.measure = varea;
+    c.w = std::sqrt(k * varea);
+    return c;
+}
+
+// Pin: reference src/constraint/pin_constraint.h:18-48, k = 1e5.
+inline Constraint make_pin(int v, const std::vector<double>& x, int row) {
+    Constraint c;
+    c.type = C_PIN;
+    c.start_row = row;
+    c.dim = 1;
+    c.k = 1e5;
+    c.inds = {v};
+    c.G = {1.0};
+    c.pin = {x[3 * v], x[3 * v + 1], x[3 * v + 2]};
+    c.w = std::sqrt(c.k);
+    c.measure = 1;
+    return c;
+}
+
+inline std::vector<Constraint> build_constraints(const SceneInput& S, const Topology& T, int* m_rows) {
+    std::vector<Constraint> cs;
+    int row = 0;
+    for (size_t f = 0; f < S.tri_faces.size(); f++) {
+        auto& fa = S.tri_faces[f];
+        if (fa[0] >= S.n_dyn) continue;
+        cs.push_back(make_tri(fa, S.x, S.tri_k[f], row));
+        row += 2;
+    }
+    for (size_t t = 0; t < S.tets.size(); t++) {
+        cs.push_back(make_tet(S.tets[t], S.x, S.tet_k[t], row));
+        row += 3;
+    }
+    for (int v = 0; v < S.n_dyn; v++) {
+        if (T.adjacency[v].size() < 3) continue;
+        cs.push_back(make_bend(v, T.adjacency[v], S.x, S.bend_k_vert[v], row));
+        row += 1;
+    }
+    for (int p : S.pins) {
+        if (p >= S.n_dyn) continue;
+        cs.push_back(make_pin(p, S.x, row));
+        row += 1;
+    }
+    *m_rows = row;
+    return cs;
+}
+
+// Lumped mass (reference src/mesh/mesh_container.h:452-472).
+inline std::vector<double> lumped_mass(const SceneInput& S) {
+    std::vector<double> m(S.n_all, 0.0);
+    for (auto& f : S.tri_faces) {
+        V3<double> a = load3(S.x.data(), f[0]), b = load3(S.x.data(), f[1]), c = load3(S.x.data(), f[2]);
+        double area = 0.5 * norm(cross(b - a, c - a));
+        for (int k = 0; k < 3; k++) m[f[k]] += area / 3.0 * S.density[f[k]];
+    }
+    for (auto& t : S.tets) {
+        V3<double> x0 = load3(S.x.data(), t[0]);
+        V3<double> a = load3(S.x.data(), t[1]) - x0, b = load3(S.x.data(), t[2]) - x0, c = load3(S.x.data(), t[3]) - x0;
+        double vol = std::fabs(dot(a, cross(b, c))) / 6.0;
+        for (int k = 0; k < 4; k++) m[t[k]] += vol / 4.0 * S.density[t[k]];
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse symmetric matrices in CSR (full pattern, sorted columns).
+// ---------------------------------------------------------------------------
+struct CSR {
+    int n = 0, m = 0; // rows, cols
+    std::vector<int> rp, ci;
+    std::vector<double> v;
+    void spmv(const double* x, double* y) const { // y = A x for 3 interleaved columns
+        for (int r = 0; r < n; r++) {
+            double s0 = 0, s1 = 0, s2 = 0;
+            for (int p = rp[r]; p < rp[r + 1]; p++) {
+                int c = ci[p];
+                s0 += v[p] * x[3 * c];
+                s1 += v[p] * x[3 * c + 1];
+                s2 += v[p] * x[3 * c + 2];
+            }
+            y[3 * r] = s0;
+            y[3 * r + 1] = s1;
+            y[3 * r + 2] = s2;
+        }
+    }
+};
+
+struct Trip {
+    int r, c;
+    double v;
+};
+
+inline CSR csr_from_triplets(int n, int m, std::vector<Trip>& t) {
+    std::sort(t.begin(), t.end(), [](const Trip& a, const Trip& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    CSR A;
+    A.n = n;
+    A.m = m;
+    A.rp.assign(n + 1, 0);
+    for (size_t i = 0; i < t.size();) {
+        size_t j = i;
+        double s = 0;
+        while (j < t.size() && t[j].r == t[i].r && t[j].c == t[i].c) s += t[j++].v;
+        A.ci.push_back(t[i].c);
+        A.v.push_back(s);
+        A.rp[t[i].r + 1]++;
+        i = j;
+    }
+    for (int r = 0; r < n; r++) A.rp[r + 1] += A.rp[r];
+    return A;
+}
+
+// ---------------------------------------------------------------------------
+// Sparse Cholesky A = L L^T with a nested-dissection ordering (separators =
+// middle BFS level of each connected piece) and the up-looking column
+// algorithm with elimination-tree reach (Davis, "Direct Methods for Sparse
+// Linear Systems", 2006, ch. 4).  Stands in for Eigen::SimplicialLLT
+// (src/solver/backend_cpu/admm_impl_cpu.cpp:248-250).
+// ---------------------------------------------------------------------------
+struct Cholesky {
+    int n = 0;
+    std::vector<int> perm, iperm; // new -> old, old -> new
+    std::vector<int> Lp, Li;
+    std::vector<double> Lx;
+    bool ok = false;
+
+    static void nd_order(const CSR& A, std::vector<int>& order) {
+        int n = A.n;
+        std::vector<int> part(n, 0); // active flag per node: current part id
+        std::vector<int> level(n, -1);
+        order.clear();
+        order.reserve(n);
+        std::vector<int> sep_stack;
+        // Work list of node sets; process depth first: separator appended after both halves.
+        struct Task {
+            std::vector<int> nodes;
+            bool emit_only;
+        };
+        std::vector<Task> stack;
+        std::vector<int> all(n);
+        std::iota(all.begin(), all.end(), 0);
+        stack.push_back({all, false});
+        std::vector<int> mark(n, 0);
+        int stamp = 1;
+        while (!stack.empty()) {
+            Task tk = std::move(stack.back());
+            stack.pop_back();
+            if (tk.emit_only || tk.nodes.size() <= 64) {
+                order.insert(order.end(), tk.nodes.begin(), tk.nodes.end());
+                continue;
+            }
+            stamp++;
+            for (int u : tk.nodes) mark[u] = stamp;
+            // BFS from the first node, then from the farthest node (pseudo-peripheral).
+            auto bfs = [&](int src, std::vector<int>& seq) {
+                seq.clear();
+                level[src] = 0;
+                seq.push_back(src);
+                mark[src] = -stamp;
+                for (size_t h = 0; h < seq.size(); h++) {
+                    int u = seq[h];
+                    for (int p = A.rp[u]; p < A.rp[u + 1]; p++) {
+                        int w = A.ci[p];
+                        if (mark[w] == stamp) {
+                            mark[w] = -stamp;
+                            level[w] = level[u] + 1;
+                            seq.push_back(w);
+                        }
+                    }
+                }
+            };
+            std::vector<int> seq;
+            bfs(tk.nodes[0], seq);
+            if (seq.size() < tk.nodes.size()) {
+                // Disconnected: split off this component, requeue the rest.
+                std::vector<int> rest;
+                for (int u : tk.nodes)
+                    if (mark[u] == stamp) rest.push_back(u);
+                stack.push_back({rest, false});
+                stack.push_back({seq, false});
+                continue;
+            }
+            int far = seq.back();
+            for (int u : seq) mark[u] = stamp;
+            bfs(far, seq);
+            int depth = level[seq.back()];
+            if (depth < 2) {
+                order.insert(order.end(), tk.nodes.begin(), tk.nodes.end());
+                continue;
+            }
+            std::vector<int> cnt(depth + 1, 0);
+            for (int u : seq) cnt[level[u]]++;
+            int half = (int)seq.size() / 2, acc = 0, mid = 1;
+            for (int l = 0; l <= depth; l++) {
+                acc += cnt[l];
+                if (acc >= half) {
+                    mid = std::min(std::max(l, 1), depth - 1);
+                    break;
+                }
+            }
+            std::vector<int> lo, hi, sep;
+            for (int u : seq) {
+                if (level[u] < mid) lo.push_back(u);
+                else if (level[u] > mid) hi.push_back(u);
+                else sep.push_back(u);
+            }
+            stack.push_back({sep, true});
+            stack.push_back({hi, false});
+            stack.push_back({lo, false});
+        }
+    }
+
+    bool factor(const CSR& A) {
+        n = A.n;
+        nd_order(A, perm);
+        iperm.assign(n, 0);
+        for (int i = 0; i < n; i++) iperm[perm[i]] = i;
+        // Permuted matrix C = P A P^T, rows hold (col, val) with col <= row (upper part of column).
+        std::vector<int> Cp(n + 1, 0), Ci;
+        std::vector<double> Cx;
+        {
+            std::vector<std::vector<std::pair<int, double>>> rows(n);
+            for (int r = 0; r < n; r++)
+                for (int p = A.rp[r]; p < A.rp[r + 1]; p++) {
+                    int i = iperm[r], j = iperm[A.ci[p]];
+                    if (j <= i) rows[i].push_back({j, A.v[p]});
+                }
+            for (int i = 0; i < n; i++) {
+                std::sort(rows[i].begin(), rows[i].end());
+                for (auto& e : rows[i]) {
+                    Ci.push_back(e.first);
+                    Cx.push_back(e.second);
+                }
+                Cp[i + 1] = (int)Ci.size();
+            }
+        }
+        // Elimination tree (row k's entries j < k).
+        std::vector<int> parent(n, -1), ancestor(n, -1);
+        for (int k = 0; k < n; k++)
+            for (int p = Cp[k]; p < Cp[k + 1]; p++) {
+                int i = Ci[p];
+                while (i != -1 && i < k) {
+                    int nx = ancestor[i];
+                    ancestor[i] = k;
+                    if (nx == -1) parent[i] = k;
+                    i = nx;
+                }
+            }
+        // Symbolic: column counts via row patterns (ereach).
+        std::vector<int> flag(n, -1), stk(n), cnt(n, 1);
+        auto reach = [&](int k, std::vector<int>& out) {
+            out.clear();
+            flag[k] = k;
+            for (int p = Cp[k]; p < Cp[k + 1]; p++) {
+                int i = Ci[p];
+                if (i >= k) continue;
+                int len = 0;
+                while (flag[i] != k) {
+                    stk[len++] = i;
+                    flag[i] = k;
+                    i = parent[i];
+                }
+                while (len > 0) out.push_back(stk[--len]);
+            }
+        };
+        std::vector<std::vector<int>> rowpat(n);
+        std::vector<int> pat;
+        for (int k = 0; k < n; k++) {
+            reach(k, pat);
+            for (int i : pat) cnt[i]++;
+            rowpat[k] = pat;
+        }
+        Lp.assign(n + 1, 0);
+        for (int j = 0; j < n; j++) Lp[j + 1] = Lp[j] + cnt[j];
+        Li.assign(Lp[n], 0);
+        Lx.assign(Lp[n], 0.0);
+        std::vector<int> c(Lp.begin(), Lp.end() - 1);
+        std::vector<double> xk(n, 0.0);
+        // Numeric up-looking factorisation.  Row k of L: solve L(0:k,0:k) l = C(0:k,k)
+        // over the topologically ordered reach; column i keeps its diagonal first.
+        for (int k = 0; k < n; k++) {
+            for (int p = Cp[k]; p < Cp[k + 1]; p++) xk[Ci[p]] = Cx[p];
+            double d = xk[k];
+            xk[k] = 0;
+            // The reach is in reverse topological order per path; process in an
+            // order where every ancestor follows its descendants: sort ascending.
+            std::vector<int>& rp = rowpat[k];
+            std::sort(rp.begin(), rp.end());
+            for (int i : rp) {
+                double lki = xk[i] / Lx[Lp[i]];
+                xk[i] = 0;
+                for (int q = Lp[i] + 1; q < c[i]; q++) xk[Li[q]] -= Lx[q] * lki;
+                d -= lki * lki;
+                Li[c[i]] = k;
+                Lx[c[i]] = lki;
+                c[i]++;
+            }
+            if (!(d > 0)) {
+                ok = false;
+                return false;
+            }
+            Li[c[k]] = k;
+            Lx[c[k]] = std::sqrt(d);
+            c[k]++;
+        }
+        ok = true;
+        return true;
+    }
+
+    // Solve A x = b for one strided column (stride 3 over xyz).
+    void solve_col(const double* b, double* x, int col, std::vector<double>& w) const {
+        w.resize(n);
+        for (int i = 0; i < n; i++) w[i] = b[3 * perm[i] + col];
+        for (int j = 0; j < n; j++) { // L y = w
+            w[j] /= Lx[Lp[j]];
+            for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) w[Li[p]] -= Lx[p] * w[j];
+        }
+        for (int j = n - 1; j >= 0; j--) { // L^T x = y
+            for (int p = Lp[j] + 1; p < Lp[j + 1]; p++) w[j] -= Lx[p] * w[Li[p]];
+            w[j] /= Lx[Lp[j]];
+        }
+        for (int i = 0; i < n; i++) x[3 * perm[i] + col] = w[i];
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Elastic prox functions (reference: TriangleConstraint::prox
+// triangle_constraint.h:169-198, TetrahedralConstraint::prox
+// tetrahedral_constraint.h:124-156, BendingConstraint::prox
+// bending_constraint.h:358-380 with rest norm 0, PinConstraint::prox
+// pin_constraint.h:432-438).  Strain limits are off in every config.
+// ---------------------------------------------------------------------------
+
+// 3x3 symmetric eigen-decomposition by cyclic Jacobi rotations (FP64, to
+// machine precision).  a is row-major symmetric; returns eigenvalues in w and
+// eigenvectors as columns of V.
+inline void sym_eig3(const double a_in[9], double w[3], double V[9]) {
+    double a[9];
+    std::memcpy(a, a_in, sizeof(a));
+    for (int i = 0; i < 9; i++) V[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 64; sweep++) {
+        double off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+        double dg = a[0] * a[0] + a[4] * a[4] + a[8] * a[8];
+        if (off <= 1e-300 || off <= dg * 1e-34) break;
+        for (int p = 0; p < 2; p++)
+            for (int q = p + 1; q < 3; q++) {
+                double apq = a[p * 3 + q];
+                if (apq == 0) continue;
+                double app = a[p * 3 + p], aqq = a[q * 3 + q];
+                double theta = (aqq - app) / (2 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                for (int k = 0; k < 3; k++) { // A <- A J
+                    double akp = a[k * 3 + p], akq = a[k * 3 + q];
+                    a[k * 3 + p] = c * akp - s * akq;
+                    a[k * 3 + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; k++) { // A <- J^T A
+                    double apk = a[p * 3 + k], aqk = a[q * 3 + k];
+                    a[p * 3 + k] = c * apk - s * aqk;
+                    a[q * 3 + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; k++) {
+                    double vkp = V[k * 3 + p], vkq = V[k * 3 + q];
+                    V[k * 3 + p] = c * vkp - s * vkq;
+                    V[k * 3 + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    w[0] = a[0];
+    w[1] = a[4];
+    w[2] = a[8];
+}
+
+// Closest rotation to F (3x3, row-major): R = U diag(1,1,det(UV^T)) V^T with
+// F = U S V^T.  Via the symmetric eigen-decomposition of F^T F (sorted
+// descending) and Gram-Schmidt of F V with the third column u1 x u2.
+inline void closest_rotation(const double F[9], double Rout[9]) {
+    double C[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += F[k * 3 + i] * F[k * 3 + j];
+            C[i * 3 + j] = s;
+        }
+    double w[3], V[9];
+    sym_eig3(C, w, V);
+    int idx[3] = {0, 1, 2};
+    std::sort(idx, idx + 3, [&](int a, int b) { return w[a] > w[b]; });
+    double Vs[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) Vs[r * 3 + c] = V[r * 3 + idx[c]];
+    // make V a rotation
+    if (det3(Vs) < 0)
+        for (int r = 0; r < 3; r++) Vs[r * 3 + 2] = -Vs[r * 3 + 2];
+    V3<double> b[3];
+    for (int c = 0; c < 3; c++) {
+        double v[3];
+        for (int r = 0; r < 3; r++) v[r] = F[r * 3 + 0] * Vs[0 * 3 + c] + F[r * 3 + 1] * Vs[1 * 3 + c] + F[r * 3 + 2] * Vs[2 * 3 + c];
+        b[c] = mk(v[0], v[1], v[2]);
+    }
+    V3<double> u0 = normalized(b[0]);
+    V3<double> u1 = normalized(b[1] - dot(u0, b[1]) * u0);
+    V3<double> u2 = cross(u0, u1);
+    double U[9] = {u0.x, u1.x, u2.x, u0.y, u1.y, u2.y, u0.z, u1.z, u2.z};
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += U[i * 3 + k] * Vs[j * 3 + k];
+            Rout[i * 3 + j] = s;
+        }
+}
+
+// z (dim x 3 rows, row r = constraint row) -> prox(z) in place.  For tri/tet,
+// F's columns are the z rows.
+template <class R> inline void prox_tri(R* z) {
+    // F = [f0 f1] (3x2); P = F (F^T F)^{-1/2} (Stiefel polar factor); z <- (P + F)/2.
+    double f0[3] = {double(z[0]), double(z[1]), double(z[2])};
+    double f1[3] = {double(z[3]), double(z[4]), double(z[5])};
+    double a = f0[0] * f0[0] + f0[1] * f0[1] + f0[2] * f0[2];
+    double b = f0[0] * f1[0] + f0[1] * f1[1] + f0[2] * f1[2];
+    double c = f1[0] * f1[0] + f1[1] * f1[1] + f1[2] * f1[2];
+    // C^{1/2} = (C + s I) / t, s = sqrt(det C), t = sqrt(tr C + 2 s); inverse of that.
+    double s = std::sqrt(std::max(a * c - b * b, 0.0));
+    double t = std::sqrt(a + c + 2 * s);
+    double m00 = (a + s) / t, m01 = b / t, m11 = (c + s) / t;
+    double det = m00 * m11 - m01 * m01;
+    double i00 = m11 / det, i01 = -m01 / det, i11 = m00 / det;
+    for (int k = 0; k < 3; k++) {
+        double p0 = f0[k] * i00 + f1[k] * i01;
+        double p1 = f0[k] * i01 + f1[k] * i11;
+        z[k] = R(0.5 * (p0 + f0[k]));
+        z[3 + k] = R(0.5 * (p1 + f1[k]));
+    }
+}
+
+template <class R> inline void prox_tet(R* z) {
+    double F[9];
+    for (int col = 0; col < 3; col++)
+        for (int r = 0; r < 3; r++) F[r * 3 + col] = double(z[col * 3 + r]);
+    double Rm[9];
+    closest_rotation(F, Rm);
+    for (int col = 0; col < 3; col++)
+        for (int r = 0; r < 3; r++) z[col * 3 + r] = R(0.5 * (Rm[r * 3 + col] + F[r * 3 + col]));
+}
+
+// ---------------------------------------------------------------------------
+// Contacts.  Canonical set = reference ProximalQuery with no broad phase
+// (src/contact/narrow_phase.cpp:192-245, 313-400, 445-469):
+//   PT: every surface vertex v against every surface triangle f with v not in f
+//       and not (static(v) && static(f0)); accept distance <= dhat.
+//   EE: every surface edge pair i < j sharing no vertex and not
+//       (static(ei0) && static(ej0)); accept distance < dhat unless both
+//       parameters are exactly 0 or 1.
+// Order: all PT contacts sorted by (v, f), then EE sorted by (i, j).
+// ContactInfo fields follow src/contact/contact_info.h:148-177 / 116-146.
+// ---------------------------------------------------------------------------
+template <class R> struct Contact {
+    int type; // 0 = PT, 1 = EE
+    int prim0, prim1; // (v, f) or (i, j)
+    int vinds[4];
+    R bary[4];
+    V3<R> normal;
+    R h_cN;
+    R dist;
+};
+
+template <class R>
+inline Contact<R> make_contact_pt(const R* x, int v, int fidx, const std::array<int, 3>& f, R dhat, const PTResult<R>& pr) {
+    Contact<R> c;
+    c.type = 0;
+    c.prim0 = v;
+    c.prim1 = fidx;
+    c.vinds[0] = v;
+    c.vinds[1] = f[0];
+    c.vinds[2] = f[1];
+    c.vinds[3] = f[2];
+    V3<R> v1 = load3(x, f[0]), v2 = load3(x, f[1]), v3 = load3(x, f[2]);
+    V3<R> n = normalized(cross(v2 - v1, v3 - v1));
+    c.bary[0] = R(1);
+    c.bary[1] = -pr.bary[0];
+    c.bary[2] = -pr.bary[1];
+    c.bary[3] = -pr.bary[2];
+    V3<R> h = pr.closest[0] - pr.closest[1];
+    R hn = dot(n, h);
+    if (hn < R(0)) {
+        n = R(-1) * n;
+        hn = -hn;
+    }
+    c.normal = n;
+    c.h_cN = hn - dhat;
+    c.dist = pr.distance;
+    return c;
+}
+
+template <class R>
+inline Contact<R> make_contact_ee(const R* x, int i, int j, const std::array<int, 2>& ei, const std::array<int, 2>& ej, R dhat,
+                                  const EEResult<R>& er) {
+    Contact<R> c;
+    c.type = 1;
+    c.prim0 = i;
+    c.prim1 = j;
+    c.vinds[0] = ei[0];
+    c.vinds[1] = ei[1];
+    c.vinds[2] = ej[0];
+    c.vinds[3] = ej[1];
+    V3<R> v0 = load3(x, ei[0]), v1 = load3(x, ei[1]), v2 = load3(x, ej[0]), v3 = load3(x, ej[1]);
+    V3<R> n = normalized(cross(v1 - v0, v3 - v2));
+    c.bary[0] = R(1) - er.param[0];
+    c.bary[1] = er.param[0];
+    c.bary[2] = er.param[1] - R(1);
+    c.bary[3] = -er.param[1];
+    V3<R> h = er.closest[0] - er.closest[1];
+    R hn = dot(n, h);
+    if (hn < R(0)) {
+        n = R(-1) * n;
+        hn = -hn;
+    }
+    c.normal = n;
+    c.h_cN = hn - dhat;
+    c.dist = er.distance;
+    return c;
+}
+
+template <class R> inline bool ee_accept(const EEResult<R>& er, R dhat) {
+    if (!(er.distance < dhat)) return false;
+    bool s_end = er.param[0] == R(0) || er.param[0] == R(1);
+    bool t_end = er.param[1] == R(0) || er.param[1] == R(1);
+    return !(s_end && t_end);
+}
+
+// Conservative axis-aligned box prefilter: a pair whose boxes are separated by
+// more than dhat on some axis cannot satisfy distance <= dhat (the distance is
+// at least the axis gap).  The margin keeps the rejection strictly outside the
+// acceptance region so the result equals the all-pairs loop.
+template <class R> struct Box {
+    R lo[3], hi[3];
+};
+
+template <class R> inline bool boxes_far(const Box<R>& a, const Box<R>& b, R gap) {
+    for (int k = 0; k < 3; k++) {
+        if (a.lo[k] - b.hi[k] > gap) return true;
+        if (b.lo[k] - a.hi[k] > gap) return true;
+    }
+    return false;
+}
+
+template <class R>
+std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute_force) {
+    const int nv = (int)T.surface_vinds.size();
+    const int nf = (int)T.surface_tris.size();
+    const int ne = (int)T.surface_edges.size();
+    std::vector<Box<R>> fb(nf), eb(ne);
+    auto boxof = [&](const int* ids, int k) {
+        Box<R> b;
+        for (int a = 0; a < 3; a++) {
+            b.lo[a] = std::numeric_limits<R>::infinity();
+            b.hi[a] = -std::numeric_limits<R>::infinity();
+        }
+        for (int q = 0; q < k; q++) {
+            V3<R> p = load3(x, ids[q]);
+            R c[3] = {p.x, p.y, p.z};
+            for (int a = 0; a < 3; a++) {
+                b.lo[a] = std::min(b.lo[a], c[a]);
+                b.hi[a] = std::max(b.hi[a], c[a]);
+            }
+        }
+        return b;
+    };
+    for (int f = 0; f < nf; f++) fb[f] = boxof(T.surface_tris[f].data(), 3);
+    for (int e = 0; e < ne; e++) eb[e] = boxof(T.surface_edges[e].data(), 2);
+    const R gap = dhat * R(1.001) + R(1e-12);
+    std::vector<std::vector<Contact<R>>> pt_out(nv), ee_out(ne);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int qi = 0; qi < nv; qi++) {
+        int v = T.surface_vinds[qi];
+        V3<R> p = load3(x, v);
+        Box<R> pb;
+        pb.lo[0] = pb.hi[0] = p.x;
+        pb.lo[1] = pb.hi[1] = p.y;
+        pb.lo[2] = pb.hi[2] = p.z;
+        for (int f = 0; f < nf; f++) {
+            const auto& fa = T.surface_tris[f];
+            if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
+            if (T.is_static[v] && T.is_static[fa[0]]) continue;
+            if (!brute_force && boxes_far(pb, fb[f], gap)) continue;
+            PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
+            if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
+        }
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < ne; i++) {
+        const auto& e0 = T.surface_edges[i];
+        for (int j = i + 1; j < ne; j++) {
+            const auto& e1 = T.surface_edges[j];
+            if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
+            if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
+            if (!brute_force && boxes_far(eb[i], eb[j], gap)) continue;
+            EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
+            if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
+        }
+    }
+    std::vector<Contact<R>> out;
+    for (auto& l : pt_out) out.insert(out.end(), l.begin(), l.end());
+    for (auto& l : ee_out) out.insert(out.end(), l.begin(), l.end());
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Solver configuration (mirrors ADU::ADMMSolverConfig, src/solver/core/admm_config.h:13-55,
+// plus the tolerances this build adds).
+// ---------------------------------------------------------------------------
+struct Config {
+    double dt = 1.0 / 60;
+    double g = 9.81;
+    int admm_max_iter = 25;
+    int dcd_interval = 5;
+    int gs_max_iter = 10;
+    int warmstart_ue = 1, warmstart_uc = 1;
+    int enable_contact = 1;
+    double mu = 0.5, kappa = 1e6, beta = 1e5;
+    double thickness = 1e-3;
+    double damp_kl = 1e-4, damp_km = 0.0;
+    double w_scale = 50.0;
+    double residual_tol = 0.0; // 0: run admm_max_iter iterations
+    int residual_interval = 1;
+};
+
+struct StepStats {
+    int iterations = 0;
+    int n_contacts = 0;
+    double r_primal = 0, r_dual = 0;
+    int converged = 0;
+};
+
+// ---------------------------------------------------------------------------
+// The ADMM solver (reference: ADMMImplCPU::step_fast,
+// src/solver/backend_cpu/admm_impl_cpu.cpp:264-359, with the Jacobi-weighted
+// contact projection of the GPU backend, src/solver/contact_solver.cu:188-431,
+// and damping included in both the matrix and the right-hand side).
+// ---------------------------------------------------------------------------
+template <class R> struct Solver {
+    SceneInput S;
+    Topology T;
+    Config cfg;
+    std::vector<Constraint> cons;
+    int m = 0, n = 0, n_all = 0;
+    std::vector<double> mass;
+    std::vector<double> wc, wc_inv; // per vertex (n_all)
+    std::vector<uint8_t> pinned;
+    CSR D, A;       // D: m x n (dynamic columns); A = M + h^2 D^T We^2 D + h^2 Wc + Damp
+    CSR Damp;
+    Cholesky llt;
+    std::vector<double> Dw; // row weights
+    // state
+    std::vector<R> x, v, Ue, Uc;
+    std::vector<Contact<R>> contacts;
+    std::vector<R> r_c;
+    std::vector<double> residual_log; // r_p, r_d per check
+    bool brute_force = false;
+
+    void setup(const SceneInput& in, const Config& c) {
+        S = in;
+        cfg = c;
+        n_all = S.n_all;
+        n = S.n_dyn;
+        T = build_topology(S.n_all, S.n_dyn, [&] { return S.tri_faces; }(), S.tet_boundary);
+        cons = build_constraints(S, T, &m);
+        mass = lumped_mass(S);
+        pinned.assign(n_all, 0);
+        for (int p : S.pins) pinned[p] = 1;
+        const double h = cfg.dt, h2 = h * h;
+        // contact weights (src/solver/solver_config_applier.cpp:34-47)
+        wc.assign(n_all, 1e12);
+        wc_inv.assign(n_all, 0.0);
+        for (int i = 0; i < n; i++) {
+            wc[i] = mass[i] / (h * h) * cfg.w_scale;
+            wc_inv[i] = 1.0 / wc[i];
+        }
+        std::vector<Trip> dt, at, dmp;
+        Dw.assign(m, 0.0);
+        for (auto& ct : cons) {
+            int k = (int)ct.inds.size();
+            for (int r = 0; r < ct.dim; r++) {
+                Dw[ct.start_row + r] = ct.w;
+                for (int j = 0; j < k; j++) dt.push_back({ct.start_row + r, ct.inds[j], ct.G[r * k + j]});
+            }
+            for (int r = 0; r < ct.dim; r++)
+                for (int a = 0; a < k; a++)
+                    for (int b = 0; b < k; b++) {
+                        double ga = ct.G[r * k + a], gb = ct.G[r * k + b];
+                        at.push_back({ct.inds[a], ct.inds[b], h2 * ct.w * ct.w * ga * gb});
+                        if (ct.type != C_PIN) dmp.push_back({ct.inds[a], ct.inds[b], h * cfg.damp_kl * ct.w * ga * gb});
+                    }
+        }
+        D = csr_from_triplets(m, n, dt);
+        std::vector<uint8_t> surf(n_all, 0);
+        for (int vv : T.surface_vinds) surf[vv] = 1;
+        for (int i = 0; i < n; i++) {
+            at.push_back({i, i, mass[i]});
+            if (cfg.enable_contact && surf[i]) at.push_back({i, i, h2 * wc[i]});
+            if (cfg.damp_km != 0) dmp.push_back({i, i, h * cfg.damp_km * mass[i]});
+        }
+        std::vector<Trip> dmp_copy = dmp;
+        Damp = csr_from_triplets(n, n, dmp_copy);
+        at.insert(at.end(), dmp.begin(), dmp.end());
+        A = csr_from_triplets(n, n, at);
+        if (!llt.factor(A)) throw std::runtime_error("LLT failed: system matrix not SPD");
+        x.assign(S.x.begin(), S.x.end());
+        v.assign(3 * n_all, R(0));
+        Ue.assign(3 * m, R(0));
+        Uc.assign(3 * n, R(0));
+        surface_flag = surf;
+    }
+    std::vector<uint8_t> surface_flag;
+
+    // Elastic local step: z = prox(D x + Ue) (src/solver/backend_cpu/local_projector_cpu.h:17-34).
+    void project_elastic(const std::vector<R>& xc, std::vector<R>& z) {
+        const int mm = m;
+        (void)mm;
+#pragma omp parallel for schedule(static)
+        for (int ci = 0; ci < (int)cons.size(); ci++) {
+            const Constraint& ct = cons[ci];
+            int k = (int)ct.inds.size();
+            R blk[9];
+            for (int r = 0; r < ct.dim; r++) {
+                R s[3] = {0, 0, 0};
+                for (int j = 0; j < k; j++) {
+                    R g = R(ct.G[r * k + j]);
+                    int vj = ct.inds[j];
+                    s[0] += g * xc[3 * vj];
+                    s[1] += g * xc[3 * vj + 1];
+                    s[2] += g * xc[3 * vj + 2];
+                }
+                int row = ct.start_row + r;
+                for (int a = 0; a < 3; a++) blk[r * 3 + a] = s[a] + Ue[3 * row + a];
+            }
+            if (ct.type == C_TRI) prox_tri(blk);
+            else if (ct.type == C_TET) prox_tet(blk);
+            else if (ct.type == C_BEND) {
+                for (int a = 0; a < 3; a++) blk[a] = blk[a] / R(2);
+            } else {
+                for (int a = 0; a < 3; a++) blk[a] = R(ct.pin[a]);
+            }
+            for (int r = 0; r < ct.dim; r++)
+                for (int a = 0; a < 3; a++) z[3 * (ct.start_row + r) + a] = blk[r * 3 + a];
+        }
+    }
+
+    // Dx (m x 3) for dynamic x.
+    void apply_D(const std::vector<R>& xc, std::vector<R>& out) {
+        out.assign(3 * m, R(0));
+#pragma omp parallel for schedule(static)
+        for (int r = 0; r < m; r++) {
+            R s0 = 0, s1 = 0, s2 = 0;
+            for (int p = D.rp[r]; p < D.rp[r + 1]; p++) {
+                R g = R(D.v[p]);
+                int c = D.ci[p];
+                s0 += g * xc[3 * c];
+                s1 += g * xc[3 * c + 1];
+                s2 += g * xc[3 * c + 2];
+            }
+            out[3 * r] = s0;
+            out[3 * r + 1] = s1;
+            out[3 * r + 2] = s2;
+        }
+    }
+
+    // out = h^2 D^T We^2 y  (n x 3)
+    void apply_DTW2(const std::vector<R>& y, std::vector<R>& out) {
+        const double h2 = cfg.dt * cfg.dt;
+        std::vector<double> acc(3 * n, 0.0);
+        for (int r = 0; r < m; r++) {
+            double s = h2 * Dw[r] * Dw[r];
+            for (int p = D.rp[r]; p < D.rp[r + 1]; p++) {
+                int c = D.ci[p];
+                double g = D.v[p] * s;
+                acc[3 * c] += g * double(y[3 * r]);
+                acc[3 * c + 1] += g * double(y[3 * r + 1]);
+                acc[3 * c + 2] += g * double(y[3 * r + 2]);
+            }
+        }
+        out.resize(3 * n);
+        for (int i = 0; i < 3 * n; i++) out[i] = R(acc[i]);
+    }
+
+    // Per-contact data for the projection (src/solver/contact_solver.cu:188-227).
+    std::vector<R> Gamma; // 4 per contact
+    std::vector<V3<R>> Kc;
+    std::vector<int> vc_start, vc_cid, vc_slot; // vertex -> (contact, slot) CSR over dynamic vertices
+
+    void compute_scc(R eps, R gam, R hinv) {
+        const int nc = (int)contacts.size();
+        std::vector<int> cnt(n_all, 0);
+        for (auto& c : contacts)
+            for (int k = 0; k < 4; k++) cnt[c.vinds[k]]++;
+        Gamma.assign(4 * nc, R(0));
+        Kc.assign(nc, V3<R>{0, 0, 0});
+        for (int ci = 0; ci < nc; ci++) {
+            const auto& c = contacts[ci];
+            R Sc = eps;
+            for (int k = 0; k < 4; k++) {
+                int vi = c.vinds[k];
+                Sc += c.bary[k] * c.bary[k] * R(wc_inv[vi]) * R(cnt[vi]);
+            }
+            for (int k = 0; k < 4; k++) {
+                int vi = c.vinds[k];
+                Gamma[4 * ci + k] = vi < n ? c.bary[k] / (Sc * R(wc[vi])) : R(0);
+            }
+            Kc[ci] = (hinv * c.h_cN * gam) * c.normal;
+        }
+        // vertex -> contact lists in contact order (deterministic)
+        vc_start.assign(n + 1, 0);
+        for (auto& c : contacts)
+            for (int k = 0; k < 4; k++)
+                if (c.vinds[k] < n) vc_start[c.vinds[k] + 1]++;
+        for (int i = 0; i < n; i++) vc_start[i + 1] += vc_start[i];
+        vc_cid.assign(vc_start[n], 0);
+        vc_slot.assign(vc_start[n], 0);
+        std::vector<int> fill(vc_start.begin(), vc_start.end() - 1);
+        for (int ci = 0; ci < nc; ci++)
+            for (int k = 0; k < 4; k++) {
+                int vi = contacts[ci].vinds[k];
+                if (vi < n) {
+                    vc_cid[fill[vi]] = ci;
+                    vc_slot[fill[vi]] = k;
+                    fill[vi]++;
+                }
+            }
+        r_c.assign(3 * nc, R(0));
+    }
+
+    // Jacobi-weighted friction-cone projection (src/solver/contact_solver.cu:316-431;
+    // per-contact formula as ADMMImplCPU::_project_one_contact, admm_impl_cpu.cpp:411-448).
+    void project_contacts(std::vector<R>& p) {
+        const int nc = (int)contacts.size();
+        const R mu = R(cfg.mu);
+        // p += Gamma r_c (gather per vertex, contact order)
+        auto gather = [&](const std::vector<R>& du) {
+#pragma omp parallel for schedule(static)
+            for (int vi = 0; vi < n; vi++) {
+                R s0 = 0, s1 = 0, s2 = 0;
+                for (int q = vc_start[vi]; q < vc_start[vi + 1]; q++) {
+                    int ci = vc_cid[q];
+                    R gmm = Gamma[4 * ci + vc_slot[q]];
+                    s0 += du[3 * ci] * gmm;
+                    s1 += du[3 * ci + 1] * gmm;
+                    s2 += du[3 * ci + 2] * gmm;
+                }
+                p[3 * vi] += s0;
+                p[3 * vi + 1] += s1;
+                p[3 * vi + 2] += s2;
+            }
+        };
+        gather(r_c);
+        std::vector<R> du(3 * nc, R(0));
+        for (int it = 0; it < cfg.gs_max_iter; it++) {
+#pragma omp parallel for schedule(static)
+            for (int ci = 0; ci < nc; ci++) {
+                const auto& c = contacts[ci];
+                V3<R> u = {0, 0, 0};
+                for (int k = 0; k < 4; k++) u = u + c.bary[k] * load3(p.data(), c.vinds[k]);
+                u = u + Kc[ci];
+                V3<R> rc = load3(r_c.data(), ci);
+                V3<R> us = u - rc;
+                R uN = dot(c.normal, us);
+                V3<R> uT = us - uN * c.normal;
+                if (uN < R(0)) {
+                    R tau = norm(uT);
+                    R alpha = -mu * uN;
+                    uN = R(0);
+                    if (tau <= alpha) uT = V3<R>{0, 0, 0};
+                    else uT = (R(1) - alpha / tau) * uT;
+                }
+                V3<R> unew = uN * c.normal + uT;
+                V3<R> d = unew - u;
+                store3(du.data(), ci, d);
+                store3(r_c.data(), ci, rc + d);
+            }
+            gather(du);
+        }
+    }
+
+    StepStats step() {
+        StepStats st;
+        const R h = R(cfg.dt), hinv = R(1) / h;
+        const double hd = cfg.dt;
+        const R eps = R(1.0 / (hd * hd * cfg.kappa + hd * cfg.beta));
+        const R gam = R(hd * cfg.kappa / (hd * cfg.kappa + cfg.beta));
+        std::vector<R> x0 = x, v0 = v;
+        for (int i = 0; i < n; i++) {
+            if (pinned[i]) {
+                v0[3 * i] = v0[3 * i + 1] = v0[3 * i + 2] = R(0);
+            } else {
+                v0[3 * i + 1] -= R(cfg.g) * h;
+            }
+        }
+        std::vector<R> xt(3 * n), Mxt(3 * n);
+        for (int i = 0; i < 3 * n; i++) {
+            xt[i] = x0[i] + h * v0[i];
+            Mxt[i] = R(mass[i / 3]) * xt[i];
+        }
+        std::vector<R> xc = x0;
+        for (int i = 0; i < 3 * n; i++) xc[i] = xt[i];
+        // b_ini = Damp x0
+        std::vector<double> x0d(3 * n), bini(3 * n);
+        for (int i = 0; i < 3 * n; i++) x0d[i] = double(x0[i]);
+        Damp.spmv(x0d.data(), bini.data());
+        std::vector<R> p = v0;
+        if (!cfg.warmstart_ue) std::fill(Ue.begin(), Ue.end(), R(0));
+        if (!cfg.warmstart_uc) std::fill(Uc.begin(), Uc.end(), R(0));
+        std::vector<R> z(3 * m, R(0)), zprev(3 * m, R(0)), DX, t1(3 * n), b(3 * n), tmp;
+        bool have_contacts = false;
+        int it = 0;
+        for (; it < cfg.admm_max_iter; it++) {
+            if (cfg.enable_contact && (it % cfg.dcd_interval == 0)) {
+                contacts = detect<R>(T, xc.data(), R(cfg.thickness), brute_force);
+                compute_scc(eps, gam, hinv);
+                have_contacts = true;
+            }
+            zprev = z;
+            project_elastic(xc, z);
+            if (cfg.enable_contact) {
+                for (int i = 0; i < 3 * n; i++) p[i] = (xc[i] - x0[i] + Uc[i]) * hinv;
+                if (have_contacts) project_contacts(p);
+            }
+            // b = M x~ + h^2 D^T We^2 (z - Ue) + h^2 Wc (h p + x0 - Uc) + b_ini
+            tmp.resize(3 * m);
+            for (int i = 0; i < 3 * m; i++) tmp[i] = z[i] - Ue[i];
+            apply_DTW2(tmp, t1);
+            const double h2 = hd * hd;
+            for (int i = 0; i < n; i++)
+                for (int a = 0; a < 3; a++) {
+                    int q = 3 * i + a;
+                    double bb = double(Mxt[q]) + double(t1[q]);
+                    if (cfg.enable_contact && surface_flag[i])
+                        bb += h2 * wc[i] * (double(h * p[q]) + double(x0[q]) - double(Uc[q]));
+                    bb += bini[q];
+                    b[q] = R(bb);
+                }
+            // x = A^{-1} b (three columns)
+            {
+                std::vector<double> bd(3 * n), xd(3 * n);
+                for (int i = 0; i < 3 * n; i++) bd[i] = double(b[i]);
+#pragma omp parallel for num_threads(3)
+                for (int col = 0; col < 3; col++) {
+                    std::vector<double> w;
+                    llt.solve_col(bd.data(), xd.data(), col, w);
+                }
+                for (int i = 0; i < 3 * n; i++) xc[i] = R(xd[i]);
+            }
+            apply_D(xc, DX);
+            for (int i = 0; i < 3 * m; i++) Ue[i] += DX[i] - z[i];
+            if (cfg.enable_contact)
+                for (int i = 0; i < 3 * n; i++) Uc[i] += xc[i] - x0[i] - h * p[i];
+            if (cfg.residual_tol > 0 && ((it + 1) % cfg.residual_interval == 0)) {
+                double rp, rd;
+                residual(xc, x0, p, z, zprev, DX, &rp, &rd);
+                st.r_primal = rp;
+                st.r_dual = rd;
+                residual_log.push_back(rp);
+                residual_log.push_back(rd);
+                if (std::max(rp, rd) <= cfg.residual_tol) {
+                    st.converged = 1;
+                    it++;
+                    break;
+                }
+            }
+        }
+        st.iterations = it;
+        st.n_contacts = (int)contacts.size();
+        for (int i = 0; i < 3 * n; i++) v[i] = (xc[i] - x0[i]) / h;
+        for (int i = 3 * n; i < 3 * n_all; i++) v[i] = v0[i];
+        x = xc;
+        return st;
+    }
+
+    // Residuals (defined by this build; the reference computes none):
+    //   r_p = sqrt(|We(Dx - z)|^2 + |Wc^{1/2}(x - x0 - h p)|^2) / max(1, sqrt(|We D x|^2 + |Wc^{1/2}(x - x0)|^2))
+    //   r_d = |h^2 D^T We^2 (z - z_prev)| / max(1, |h^2 D^T We^2 z|)
+    void residual(const std::vector<R>& xc, const std::vector<R>& x0, const std::vector<R>& p, const std::vector<R>& z,
+                  const std::vector<R>& zprev, const std::vector<R>& DX, double* rp, double* rd) {
+        double num = 0, den = 0;
+        for (int r = 0; r < m; r++) {
+            double w2 = Dw[r] * Dw[r];
+            for (int a = 0; a < 3; a++) {
+                double d = double(DX[3 * r + a]) - double(z[3 * r + a]);
+                num += w2 * d * d;
+                den += w2 * double(DX[3 * r + a]) * double(DX[3 * r + a]);
+            }
+        }
+        if (cfg.enable_contact)
+            for (int i = 0; i < n; i++) {
+                if (!surface_flag[i]) continue;
+                for (int a = 0; a < 3; a++) {
+                    double dx = double(xc[3 * i + a]) - double(x0[3 * i + a]);
+                    double d = dx - cfg.dt * double(p[3 * i + a]);
+                    num += wc[i] * d * d;
+                    den += wc[i] * dx * dx;
+                }
+            }
+        *rp = std::sqrt(num) / std::max(1.0, std::sqrt(den));
+        std::vector<R> dz(3 * m), o1, o2;
+        for (int i = 0; i < 3 * m; i++) dz[i] = z[i] - zprev[i];
+        apply_DTW2(dz, o1);
+        apply_DTW2(z, o2);
+        double a1 = 0, a2 = 0;
+        for (int i = 0; i < 3 * n; i++) {
+            a1 += double(o1[i]) * double(o1[i]);
+            a2 += double(o2[i]) * double(o2[i]);
+        }
+        *rd = std::sqrt(a1) / std::max(1.0, std::sqrt(a2));
+    }
+};
+
+} // namespace orc

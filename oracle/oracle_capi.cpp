@@ -1,0 +1,370 @@
+// C entry points of the FP64 oracle (TEST INFRASTRUCTURE ONLY: loaded by
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg through
+// ctypes; never linked by the product library).
+#include "oracle.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <memory>
+
+using namespace orc;
+
+namespace {
+
+struct Handle {
+    int prec = 64;
+    std::unique_ptr<Solver<double>> d;
+    std::unique_ptr<Solver<float>> f;
+    std::string err;
+    double last_step_ms = 0;
+};
+
+SceneInput make_scene(int n_all, int n_dyn, const double* x, const double* density, int n_tri, const int* tris,
+                      const double* tri_k, const double* bend_k_vert, int n_tet, const int* tets, const double* tet_k,
+                      int n_tb, const int* tet_boundary, int n_pins, const int* pins) {
+    SceneInput S;
+    S.n_all = n_all;
+    S.n_dyn = n_dyn;
+    S.x.assign(x, x + 3 * (size_t)n_all);
+    S.density.assign(density, density + n_all);
+    for (int i = 0; i < n_tri; i++) S.tri_faces.push_back({tris[3 * i], tris[3 * i + 1], tris[3 * i + 2]});
+    S.tri_k.assign(tri_k, tri_k + n_tri);
+    S.bend_k_vert.assign(bend_k_vert, bend_k_vert + n_all);
+    for (int i = 0; i < n_tet; i++) S.tets.push_back({tets[4 * i], tets[4 * i + 1], tets[4 * i + 2], tets[4 * i + 3]});
+    S.tet_k.assign(tet_k, tet_k + n_tet);
+    for (int i = 0; i < n_tb; i++)
+        S.tet_boundary.push_back({tet_boundary[3 * i], tet_boundary[3 * i + 1], tet_boundary[3 * i + 2]});
+    S.pins.assign(pins, pins + n_pins);
+    return S;
+}
+
+template <class R, class F> auto with(Handle* h, F&& fn) {
+    if constexpr (std::is_same_v<R, double>) return fn(*h->d);
+    else return fn(*h->f);
+}
+
+} // namespace
+
+extern "C" {
+
+struct orc_config {
+    double dt, g;
+    int admm_max_iter, dcd_interval, gs_max_iter, warmstart_ue, warmstart_uc, enable_contact;
+    double mu, kappa, beta, thickness, damp_kl, damp_km, w_scale, residual_tol;
+    int residual_interval, brute_force;
+};
+
+struct orc_stats {
+    int iterations, n_contacts, converged, pad;
+    double r_primal, r_dual, step_ms;
+};
+
+void* orc_create(int prec, int n_all, int n_dyn, const double* x, const double* density, int n_tri, const int* tris,
+                 const double* tri_k, const double* bend_k_vert, int n_tet, const int* tets, const double* tet_k, int n_tb,
+                 const int* tet_boundary, int n_pins, const int* pins, const orc_config* c, char* err, int err_len) {
+    try {
+        SceneInput S = make_scene(n_all, n_dyn, x, density, n_tri, tris, tri_k, bend_k_vert, n_tet, tets, tet_k, n_tb,
+                                  tet_boundary, n_pins, pins);
+        Config cfg;
+        cfg.dt = c->dt;
+        cfg.g = c->g;
+        cfg.admm_max_iter = c->admm_max_iter;
+        cfg.dcd_interval = c->dcd_interval;
+        cfg.gs_max_iter = c->gs_max_iter;
+        cfg.warmstart_ue = c->warmstart_ue;
+        cfg.warmstart_uc = c->warmstart_uc;
+        cfg.enable_contact = c->enable_contact;
+        cfg.mu = c->mu;
+        cfg.kappa = c->kappa;
+        cfg.beta = c->beta;
+        cfg.thickness = c->thickness;
+        cfg.damp_kl = c->damp_kl;
+        cfg.damp_km = c->damp_km;
+        cfg.w_scale = c->w_scale;
+        cfg.residual_tol = c->residual_tol;
+        cfg.residual_interval = c->residual_interval > 0 ? c->residual_interval : 1;
+        auto* h = new Handle;
+        h->prec = prec;
+        if (prec == 32) {
+            h->f = std::make_unique<Solver<float>>();
+            h->f->brute_force = c->brute_force != 0;
+            h->f->setup(S, cfg);
+        } else {
+            h->d = std::make_unique<Solver<double>>();
+            h->d->brute_force = c->brute_force != 0;
+            h->d->setup(S, cfg);
+        }
+        return h;
+    } catch (const std::exception& e) {
+        if (err && err_len > 0) std::snprintf(err, err_len, "%s", e.what());
+        return nullptr;
+    }
+}
+
+void orc_destroy(void* hp) { delete static_cast<Handle*>(hp); }
+
+#define DISPATCH(h, ...)                                                                                                \
+    do {                                                                                                                 \
+        if ((h)->prec == 32) {                                                                                           \
+            auto& s = *(h)->f;                                                                                           \
+            using R = float;                                                                                             \
+            (void)sizeof(R);                                                                                             \
+            __VA_ARGS__;                                                                                                 \
+        } else {                                                                                                         \
+            auto& s = *(h)->d;                                                                                           \
+            using R = double;                                                                                            \
+            (void)sizeof(R);                                                                                             \
+            __VA_ARGS__;                                                                                                 \
+        }                                                                                                                \
+    } while (0)
+
+int orc_step(void* hp, orc_stats* out) {
+    auto* h = static_cast<Handle*>(hp);
+    auto t0 = std::chrono::steady_clock::now();
+    StepStats st;
+    DISPATCH(h, st = s.step());
+    double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out) {
+        out->iterations = st.iterations;
+        out->n_contacts = st.n_contacts;
+        out->converged = st.converged;
+        out->r_primal = st.r_primal;
+        out->r_dual = st.r_dual;
+        out->step_ms = ms;
+    }
+    return 0;
+}
+
+// sizes: n_all, n_dyn, m, nnz_D, nnz_A, n_surf_v, n_surf_t, n_surf_e, n_constraints, n_contacts, nnz_L
+void orc_sizes(void* hp, long long* out) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        out[0] = s.n_all;
+        out[1] = s.n;
+        out[2] = s.m;
+        out[3] = (long long)s.D.ci.size();
+        out[4] = (long long)s.A.ci.size();
+        out[5] = (long long)s.T.surface_vinds.size();
+        out[6] = (long long)s.T.surface_tris.size();
+        out[7] = (long long)s.T.surface_edges.size();
+        out[8] = (long long)s.cons.size();
+        out[9] = (long long)s.contacts.size();
+        out[10] = (long long)s.llt.Lp.back();
+    });
+}
+
+void orc_get_state(void* hp, double* x, double* v) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (int i = 0; i < 3 * s.n_all; i++) {
+            if (x) x[i] = double(s.x[i]);
+            if (v) v[i] = double(s.v[i]);
+        }
+    });
+}
+
+void orc_set_state(void* hp, const double* x, const double* v) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (int i = 0; i < 3 * s.n_all; i++) {
+            if (x) s.x[i] = R(x[i]);
+            if (v) s.v[i] = R(v[i]);
+        }
+    });
+}
+
+// Duals: Ue (3m), Uc (3n).
+void orc_get_duals(void* hp, double* ue, double* uc) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (int i = 0; i < 3 * s.m; i++) if (ue) ue[i] = double(s.Ue[i]);
+        for (int i = 0; i < 3 * s.n; i++) if (uc) uc[i] = double(s.Uc[i]);
+    });
+}
+
+void orc_set_duals(void* hp, const double* ue, const double* uc) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (int i = 0; i < 3 * s.m; i++) if (ue) s.Ue[i] = R(ue[i]);
+        for (int i = 0; i < 3 * s.n; i++) if (uc) s.Uc[i] = R(uc[i]);
+    });
+}
+
+void orc_get_mass(void* hp, double* mass) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, { for (int i = 0; i < s.n_all; i++) mass[i] = s.mass[i]; });
+}
+
+// Topology: surface vinds (nv), tris (3 nt), edges (2 ne).
+void orc_get_topology(void* hp, int* vinds, int* tris, int* edges) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (size_t i = 0; i < s.T.surface_vinds.size(); i++) vinds[i] = s.T.surface_vinds[i];
+        for (size_t i = 0; i < s.T.surface_tris.size(); i++)
+            for (int k = 0; k < 3; k++) tris[3 * i + k] = s.T.surface_tris[i][k];
+        for (size_t i = 0; i < s.T.surface_edges.size(); i++)
+            for (int k = 0; k < 2; k++) edges[2 * i + k] = s.T.surface_edges[i][k];
+    });
+}
+
+// which: 0 = D (m x n), 1 = A (n x n), 2 = Damp (n x n).
+void orc_get_matrix(void* hp, int which, int* rp, int* ci, double* v) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        const CSR& M = which == 0 ? s.D : (which == 1 ? s.A : s.Damp);
+        for (int i = 0; i <= M.n; i++) rp[i] = M.rp[i];
+        for (size_t i = 0; i < M.ci.size(); i++) {
+            ci[i] = M.ci[i];
+            v[i] = M.v[i];
+        }
+    });
+}
+
+long long orc_matrix_nnz(void* hp, int which) {
+    auto* h = static_cast<Handle*>(hp);
+    long long r = 0;
+    DISPATCH(h, {
+        const CSR& M = which == 0 ? s.D : (which == 1 ? s.A : s.Damp);
+        r = (long long)M.ci.size();
+    });
+    return r;
+}
+
+// Row weights w (m).
+void orc_get_row_weights(void* hp, double* w) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, { for (int i = 0; i < s.m; i++) w[i] = s.Dw[i]; });
+}
+
+// Contacts of the last detection: type, prim0, prim1 (3 ints), vinds (4 ints),
+// bary (4), normal (3), h_cN, dist.
+int orc_get_contacts(void* hp, int* ints, double* reals, int cap) {
+    auto* h = static_cast<Handle*>(hp);
+    int nc = 0;
+    DISPATCH(h, {
+        nc = (int)s.contacts.size();
+        for (int i = 0; i < nc && i < cap; i++) {
+            const auto& c = s.contacts[i];
+            ints[7 * i + 0] = c.type;
+            ints[7 * i + 1] = c.prim0;
+            ints[7 * i + 2] = c.prim1;
+            for (int k = 0; k < 4; k++) ints[7 * i + 3 + k] = c.vinds[k];
+            for (int k = 0; k < 4; k++) reals[9 * i + k] = double(c.bary[k]);
+            reals[9 * i + 4] = double(c.normal.x);
+            reals[9 * i + 5] = double(c.normal.y);
+            reals[9 * i + 6] = double(c.normal.z);
+            reals[9 * i + 7] = double(c.h_cN);
+            reals[9 * i + 8] = double(c.dist);
+        }
+    });
+    return nc;
+}
+
+// Run detection on the given positions (3 n_all) and keep the result as the
+// current contact list.
+int orc_detect(void* hp, const double* x, int brute_force) {
+    auto* h = static_cast<Handle*>(hp);
+    int nc = 0;
+    DISPATCH(h, {
+        std::vector<R> xr(3 * (size_t)s.n_all);
+        for (size_t i = 0; i < xr.size(); i++) xr[i] = R(x[i]);
+        s.contacts = detect<R>(s.T, xr.data(), R(s.cfg.thickness), brute_force != 0);
+        nc = (int)s.contacts.size();
+    });
+    return nc;
+}
+
+// Contact projection on a given p (3 n_all) for the current contacts:
+// compute_scc then one project_contacts call (gs_max_iter Jacobi sweeps).
+// Returns r_c (3 nc) and updated p.
+void orc_project_contacts(void* hp, double* p, double* rc_out) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        const double hd = s.cfg.dt;
+        R eps = R(1.0 / (hd * hd * s.cfg.kappa + hd * s.cfg.beta));
+        R gam = R(hd * s.cfg.kappa / (hd * s.cfg.kappa + s.cfg.beta));
+        s.compute_scc(eps, gam, R(1) / R(hd));
+        std::vector<R> pr(3 * (size_t)s.n_all);
+        for (size_t i = 0; i < pr.size(); i++) pr[i] = R(p[i]);
+        s.project_contacts(pr);
+        for (size_t i = 0; i < pr.size(); i++) p[i] = double(pr[i]);
+        for (size_t i = 0; i < s.r_c.size(); i++) rc_out[i] = double(s.r_c[i]);
+    });
+}
+
+// Elastic local step on given x (3 n_all), Ue (3m) -> z (3m).
+void orc_project_elastic(void* hp, const double* x, const double* ue, double* z) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        std::vector<R> xr(3 * (size_t)s.n_all), zr(3 * (size_t)s.m);
+        for (size_t i = 0; i < xr.size(); i++) xr[i] = R(x[i]);
+        std::vector<R> save = s.Ue;
+        for (int i = 0; i < 3 * s.m; i++) s.Ue[i] = R(ue[i]);
+        s.project_elastic(xr, zr);
+        s.Ue = save;
+        for (size_t i = 0; i < zr.size(); i++) z[i] = double(zr[i]);
+    });
+}
+
+// Solve A x = b with the prefactored LLT (b, x: 3n interleaved).
+void orc_solve(void* hp, const double* b, double* x) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        for (int col = 0; col < 3; col++) {
+            std::vector<double> w;
+            s.llt.solve_col(b, x, col, w);
+        }
+    });
+}
+
+// Residual log (pairs r_p, r_d).
+int orc_residual_log(void* hp, double* out, int cap) {
+    auto* h = static_cast<Handle*>(hp);
+    int n = 0;
+    DISPATCH(h, {
+        n = (int)s.residual_log.size();
+        for (int i = 0; i < n && i < cap; i++) out[i] = s.residual_log[i];
+    });
+    return n;
+}
+
+// ---- stateless kernels for unit tests ----
+void orc_dist_pt(const double* p, const double* t0, const double* t1, const double* t2, double* out /*dist, sq, b0,b1,b2*/) {
+    auto r = dist_pt(load3(p, 0), load3(t0, 0), load3(t1, 0), load3(t2, 0));
+    out[0] = r.distance;
+    out[1] = r.sqr_distance;
+    out[2] = r.bary[0];
+    out[3] = r.bary[1];
+    out[4] = r.bary[2];
+}
+
+void orc_dist_ee(const double* p0, const double* p1, const double* q0, const double* q1, double* out /*dist, sq, s, t*/) {
+    auto r = dist_ee(load3(p0, 0), load3(p1, 0), load3(q0, 0), load3(q1, 0));
+    out[0] = r.distance;
+    out[1] = r.sqr_distance;
+    out[2] = r.param[0];
+    out[3] = r.param[1];
+}
+
+void orc_prox_tri(double* z6) { prox_tri(z6); }
+void orc_prox_tet(double* z9) { prox_tet(z9); }
+
+// Factor a symmetric CSR matrix and solve one RHS (for the LLT unit test).
+int orc_llt_solve(int n, const int* rp, const int* ci, const double* v, const double* b, double* x, long long* nnz_l) {
+    CSR A;
+    A.n = A.m = n;
+    A.rp.assign(rp, rp + n + 1);
+    A.ci.assign(ci, ci + rp[n]);
+    A.v.assign(v, v + rp[n]);
+    Cholesky L;
+    if (!L.factor(A)) return -1;
+    std::vector<double> b3(3 * (size_t)n, 0.0), x3(3 * (size_t)n, 0.0);
+    for (int i = 0; i < n; i++) b3[3 * i] = b[i];
+    std::vector<double> w;
+    L.solve_col(b3.data(), x3.data(), 0, w);
+    for (int i = 0; i < n; i++) x[i] = x3[3 * i];
+    if (nnz_l) *nnz_l = L.Lp.back();
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,235 @@
+"""ctypes view of the FP64 oracle (TEST INFRASTRUCTURE ONLY).
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as
+the checker; the product package never imports it.  Built by `make -C oracle`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("g", C.c_double)] + [
+        (k, C.c_int32) for k in ("admm_max_iter", "dcd_interval", "gs_max_iter", "warmstart_ue", "warmstart_uc", "enable_contact")] + [
+        (k, C.c_double) for k in ("mu", "kappa", "beta", "thickness", "damp_kl", "damp_km", "w_scale", "residual_tol")] + [
+        ("residual_interval", C.c_int32), ("brute_force", C.c_int32)]
+
+
+class OrcStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("n_contacts", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
+                ("r_primal", C.c_double), ("r_dual", C.c_double), ("step_ms", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} missing: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.orc_create.restype = C.c_void_p
+        L.orc_matrix_nnz.restype = C.c_longlong
+        for name in ("orc_destroy", "orc_step", "orc_sizes", "orc_get_state", "orc_set_state", "orc_get_duals", "orc_set_duals",
+                     "orc_get_mass", "orc_get_topology", "orc_get_matrix", "orc_matrix_nnz", "orc_get_row_weights",
+                     "orc_get_contacts", "orc_detect", "orc_project_contacts", "orc_project_elastic", "orc_solve",
+                     "orc_residual_log"):
+            getattr(L, name).argtypes = None
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def config_from(cfg, brute_force: bool = False) -> OrcConfig:
+    """Builds the oracle's config from a parallel_admm_amd.AdmmConfig-like object."""
+    c = OrcConfig()
+    for k, _ in OrcConfig._fields_:
+        if k == "brute_force":
+            c.brute_force = int(brute_force)
+        else:
+            v = getattr(cfg, k)
+            setattr(c, k, int(v) if isinstance(v, bool) else v)
+    return c
+
+
+class Oracle:
+    """FP64 (prec=64) or FP32-twin (prec=32) restatement of the ADMM step."""
+
+    def __init__(self, mesh: dict, cfg, prec: int = 64, brute_force: bool = False):
+        L = lib()
+        self.mesh = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
+        m = self.mesh
+        x = np.ascontiguousarray(m["x"], np.float64).reshape(-1)
+        n_all = x.size // 3
+        keep = dict(x=x, density=np.ascontiguousarray(m["density"], np.float64),
+                    tris=np.ascontiguousarray(m["tris"], np.int32).reshape(-1), tri_k=np.ascontiguousarray(m["tri_k"], np.float64),
+                    bend_k=np.ascontiguousarray(m["bend_k"], np.float64), tets=np.ascontiguousarray(m["tets"], np.int32).reshape(-1),
+                    tet_k=np.ascontiguousarray(m["tet_k"], np.float64),
+                    tb=np.ascontiguousarray(m["tet_boundary"], np.int32).reshape(-1), pins=np.ascontiguousarray(m["pins"], np.int32))
+        self._keep = keep
+        self.cfg = config_from(cfg, brute_force)
+        err = C.create_string_buffer(512)
+        h = L.orc_create(C.c_int(prec), C.c_int(n_all), C.c_int(int(m["n_dyn"])), _p(keep["x"]), _p(keep["density"]),
+                         C.c_int(keep["tris"].size // 3), _p(keep["tris"]), _p(keep["tri_k"]), _p(keep["bend_k"]),
+                         C.c_int(keep["tets"].size // 4), _p(keep["tets"]), _p(keep["tet_k"]), C.c_int(keep["tb"].size // 3),
+                         _p(keep["tb"]), C.c_int(keep["pins"].size), _p(keep["pins"]), C.byref(self.cfg), err, C.c_int(512))
+        if not h:
+            raise RuntimeError(f"oracle create failed: {err.value.decode()}")
+        self._h = C.c_void_p(h)
+        s = np.zeros(11, np.int64)
+        L.orc_sizes(self._h, _p(s))
+        (self.n_all, self.n_dyn, self.m, self.nnz_d, self.nnz_a, self.n_surf_v, self.n_surf_t, self.n_surf_e,
+         self.n_constraints, _, self.nnz_l) = [int(v) for v in s]
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def step(self) -> dict:
+        st = OrcStats()
+        lib().orc_step(self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in OrcStats._fields_}
+
+    def state(self):
+        x = np.zeros((self.n_all, 3))
+        v = np.zeros((self.n_all, 3))
+        lib().orc_get_state(self._h, _p(x), _p(v))
+        return x, v
+
+    def set_state(self, x=None, v=None):
+        xa = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        lib().orc_set_state(self._h, _p(xa), _p(va))
+
+    def duals(self):
+        ue = np.zeros((self.m, 3))
+        uc = np.zeros((self.n_dyn, 3))
+        lib().orc_get_duals(self._h, _p(ue), _p(uc))
+        return ue, uc
+
+    def set_duals(self, ue=None, uc=None):
+        a = None if ue is None else np.ascontiguousarray(ue, np.float64).reshape(-1)
+        b = None if uc is None else np.ascontiguousarray(uc, np.float64).reshape(-1)
+        lib().orc_set_duals(self._h, _p(a), _p(b))
+
+    def mass(self):
+        m = np.zeros(self.n_all)
+        lib().orc_get_mass(self._h, _p(m))
+        return m
+
+    def topology(self):
+        sv = np.zeros(self.n_surf_v, np.int32)
+        st = np.zeros((self.n_surf_t, 3), np.int32)
+        se = np.zeros((self.n_surf_e, 2), np.int32)
+        lib().orc_get_topology(self._h, _p(sv), _p(st), _p(se))
+        return dict(surf_verts=sv, surf_tris=st, surf_edges=se)
+
+    def matrix(self, which: int):
+        """which: 0 = D (m x n), 1 = A (n x n, damping included), 2 = Damp.  Returns scipy CSR."""
+        import scipy.sparse as sp
+        nnz = int(lib().orc_matrix_nnz(self._h, C.c_int(which)))
+        rows = self.m if which == 0 else self.n_dyn
+        rp = np.zeros(rows + 1, np.int32)
+        ci = np.zeros(nnz, np.int32)
+        v = np.zeros(nnz)
+        lib().orc_get_matrix(self._h, C.c_int(which), _p(rp), _p(ci), _p(v))
+        return sp.csr_matrix((v, ci, rp), shape=(rows, self.n_dyn))
+
+    def row_weights(self):
+        w = np.zeros(self.m)
+        lib().orc_get_row_weights(self._h, _p(w))
+        return w
+
+    def contacts(self, cap: int = 1 << 22):
+        n = self._ncontacts()
+        ints = np.zeros((max(n, 1), 7), np.int32)
+        reals = np.zeros((max(n, 1), 9))
+        got = lib().orc_get_contacts(self._h, _p(ints), _p(reals), C.c_int(max(n, 1)))
+        return ints[:got], reals[:got]
+
+    def _ncontacts(self):
+        s = np.zeros(11, np.int64)
+        lib().orc_sizes(self._h, _p(s))
+        return int(s[9])
+
+    def detect(self, x, brute_force: bool = False) -> int:
+        xa = np.ascontiguousarray(x, np.float64).reshape(-1)
+        return int(lib().orc_detect(self._h, _p(xa), C.c_int(int(brute_force))))
+
+    def project_contacts(self, p):
+        pa_ = np.ascontiguousarray(p, np.float64).reshape(-1).copy()
+        n = self._ncontacts()
+        rc = np.zeros((max(n, 1), 3))
+        lib().orc_project_contacts(self._h, _p(pa_), _p(rc))
+        return pa_.reshape(-1, 3), rc[:n]
+
+    def project_elastic(self, x, ue):
+        xa = np.ascontiguousarray(x, np.float64).reshape(-1)
+        ua = np.ascontiguousarray(ue, np.float64).reshape(-1)
+        z = np.zeros((self.m, 3))
+        lib().orc_project_elastic(self._h, _p(xa), _p(ua), _p(z))
+        return z
+
+    def solve(self, b):
+        ba = np.ascontiguousarray(b, np.float64).reshape(-1)
+        x = np.zeros((self.n_dyn, 3))
+        lib().orc_solve(self._h, _p(ba), _p(x))
+        return x
+
+    def residual_log(self):
+        buf = np.zeros(1 << 16)
+        n = lib().orc_residual_log(self._h, _p(buf), C.c_int(buf.size))
+        return buf[:n].reshape(-1, 2)
+
+
+def dist_pt(p, t0, t1, t2):
+    out = np.zeros(5)
+    a = [np.ascontiguousarray(v, np.float64) for v in (p, t0, t1, t2)]
+    lib().orc_dist_pt(*[_p(v) for v in a], _p(out))
+    return out
+
+
+def dist_ee(p0, p1, q0, q1):
+    out = np.zeros(4)
+    a = [np.ascontiguousarray(v, np.float64) for v in (p0, p1, q0, q1)]
+    lib().orc_dist_ee(*[_p(v) for v in a], _p(out))
+    return out
+
+
+def prox_tri(z6):
+    z = np.ascontiguousarray(z6, np.float64).reshape(-1).copy()
+    lib().orc_prox_tri(_p(z))
+    return z.reshape(2, 3)
+
+
+def prox_tet(z9):
+    z = np.ascontiguousarray(z9, np.float64).reshape(-1).copy()
+    lib().orc_prox_tet(_p(z))
+    return z.reshape(3, 3)
+
+
+def llt_solve(A, b):
+    """Factor symmetric CSR A (scipy) and solve A x = b; returns (x, nnz(L))."""
+    A = A.tocsr()
+    A.sort_indices()
+    x = np.zeros(A.shape[0])
+    nnz = C.c_longlong()
+    rp = np.ascontiguousarray(A.indptr, np.int32)
+    ci = np.ascontiguousarray(A.indices, np.int32)
+    v = np.ascontiguousarray(A.data, np.float64)
+    bb = np.ascontiguousarray(b, np.float64)
+    r = lib().orc_llt_solve(C.c_int(A.shape[0]), _p(rp), _p(ci), _p(v), _p(bb), _p(x), C.byref(nnz))
+    if r != 0:
+        raise RuntimeError("LLT failed")
+    return x, nnz.value

@@ -1,0 +1,476 @@
+"""MI355X (gfx950) ADMM frictional-contact backend: Python view of the C ABI.
+
+The classes mirror the reference's interfaces for this path:
+  ADMMBackendHIP    ADU::IADMMBackend + the Solver entry points it reaches
+                    through as_solver() (src/solver/core/admm_backend.h:15-41,
+                    src/solver/core/admm_solver.cpp:5-17, src/solver/solver.h:66-137)
+  CollisionDetector ADU::ICollisionDetector (src/contact/icollision_detector.h:19-47)
+  Scene             MeshData + Mesh2Constraint setup (src/mesh/mesh_container.h,
+                    src/constraint/mesh_to_constraint.h)
+Everything runs in libpa_admm.so (HIP kernels + C++ host code); this module only
+moves arrays across the C ABI.  There is no CPU fallback: a missing library or
+GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libpa_admm.so")
+
+PA_OK, PA_ERR_INVALID, PA_ERR_HIP, PA_ERR_NO_DEVICE, PA_ERR_OVERFLOW, PA_ERR_NOT_SPD, PA_ERR_GEOMETRY = range(7)
+
+# Every symbol declared in include/pa_admm.h.
+EXPORTS = [
+    "pa_admm_config_default", "pa_scene_create", "pa_scene_generate", "pa_scene_destroy", "pa_scene_get_info",
+    "pa_scene_get_mesh", "pa_scene_get_topology", "pa_scene_get_mass", "pa_scene_get_constraints",
+    "pa_detector_create", "pa_detector_destroy", "pa_detector_detect", "pa_detector_get_contacts",
+    "pa_detector_counts", "pa_detector_points", "pa_detector_normals", "pa_detector_clear",
+    "pa_backend_create", "pa_backend_destroy", "pa_backend_last_error", "pa_backend_apply_config",
+    "pa_backend_set_constraints", "pa_backend_set_constraint_dim", "pa_backend_finalize_constraints",
+    "pa_backend_init", "pa_backend_set_sync_to_host", "pa_backend_attach_detector", "pa_backend_step",
+    "pa_backend_reset", "pa_backend_get_state", "pa_backend_set_state", "pa_backend_get_duals",
+    "pa_backend_set_duals", "pa_backend_set_static_state", "pa_backend_create_from_scene",
+    "pa_backend_project_elastic", "pa_backend_project_contacts", "pa_backend_global_solve",
+    "pa_backend_set_profiling", "pa_backend_get_sizes", "pa_backend_phase_times", "pa_backend_kernel_time", "pa_last_error_message",
+]
+
+
+class PAError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[pa status {code}] {msg}")
+        self.code = code
+
+
+class AdmmConfigC(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("g", C.c_double),
+        ("admm_max_iter", C.c_int32), ("dcd_interval", C.c_int32), ("gs_max_iter", C.c_int32),
+        ("warmstart_ue", C.c_int32), ("warmstart_uc", C.c_int32), ("enable_contact", C.c_int32),
+        ("mu", C.c_double), ("kappa", C.c_double), ("beta", C.c_double), ("thickness", C.c_double),
+        ("damp_kl", C.c_double), ("damp_km", C.c_double), ("w_scale", C.c_double), ("residual_tol", C.c_double),
+        ("residual_interval", C.c_int32), ("global_max_iter", C.c_int32), ("global_tol", C.c_double),
+        ("max_contacts", C.c_int32), ("global_refine", C.c_int32),
+    ]
+
+
+class MeshDescC(C.Structure):
+    _fields_ = [
+        ("n_all", C.c_int32), ("n_dyn", C.c_int32), ("x", C.c_void_p), ("density", C.c_void_p),
+        ("n_tri", C.c_int32), ("tris", C.c_void_p), ("tri_k", C.c_void_p), ("bend_k", C.c_void_p),
+        ("n_tet", C.c_int32), ("tets", C.c_void_p), ("tet_k", C.c_void_p),
+        ("n_tet_boundary", C.c_int32), ("tet_boundary", C.c_void_p), ("n_pins", C.c_int32), ("pins", C.c_void_p),
+    ]
+
+
+class SceneInfoC(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in (
+        "n_all", "n_dyn", "m_rows", "n_tri", "n_tet", "n_tet_boundary", "n_pins",
+        "n_tri_c", "n_tet_c", "n_bend_c", "n_pin_c", "n_surf_verts", "n_surf_tris", "n_surf_edges")] + [
+        ("nnz_d", C.c_int64), ("bend_entries", C.c_int64)]
+
+
+class ConstraintSoAC(C.Structure):
+    _fields_ = [(k, C.c_int32) for k in ("n_tri", "n_tet", "n_bend", "n_pin", "tri_row0", "tet_row0", "bend_row0", "pin_row0")] + [
+        (k, C.c_void_p) for k in ("tri_inds", "tri_binv", "tri_w", "tet_inds", "tet_binv", "tet_w", "bend_ptr",
+                                  "bend_inds", "bend_coef", "bend_w", "pin_inds", "pin_pos", "pin_w")]
+
+
+class SurfaceC(C.Structure):
+    _fields_ = [("n_all", C.c_int32), ("n_dyn", C.c_int32), ("n_surf_verts", C.c_int32), ("n_surf_tris", C.c_int32),
+                ("n_surf_edges", C.c_int32), ("surf_verts", C.c_void_p), ("surf_tris", C.c_void_p), ("surf_edges", C.c_void_p)]
+
+
+class DeviceOptsC(C.Structure):
+    _fields_ = [("device", C.c_int32), ("verbose", C.c_int32)]
+
+
+class StepStatsC(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("n_contacts", C.c_int32), ("converged", C.c_int32), ("overflow", C.c_int32),
+                ("global_iters", C.c_int32), ("detections", C.c_int32), ("r_primal", C.c_double), ("r_dual", C.c_double),
+                ("step_ms", C.c_double)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Loads libpa_admm.so (built by parallel_admm_amd.build); raises if absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m parallel_admm_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        for name in EXPORTS:
+            getattr(L, name)  # every declared symbol must resolve
+        L.pa_backend_last_error.restype = C.c_char_p
+        L.pa_last_error_message.restype = C.c_char_p
+        L.pa_scene_generate.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_double, C.c_double, C.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]) -> Optional[C.c_void_p]:
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def _check(code: int, what: str = "") -> None:
+    if code != PA_OK:
+        msg = lib().pa_last_error_message().decode(errors="replace")
+        raise PAError(code, f"{what}: {msg}" if what else msg)
+
+
+@dataclasses.dataclass
+class AdmmConfig:
+    """ADMMSolverConfig (src/solver/core/admm_config.h:13-55) plus this backend's
+    convergence controls.  Defaults are the reference's."""
+    dt: float = 1.0 / 60.0
+    g: float = 0.98
+    admm_max_iter: int = 25
+    dcd_interval: int = 5
+    gs_max_iter: int = 10
+    warmstart_ue: bool = True
+    warmstart_uc: bool = True
+    enable_contact: bool = True
+    mu: float = 0.5
+    kappa: float = 1e6
+    beta: float = 1e5
+    thickness: float = 5e-2
+    damp_kl: float = 1e-4
+    damp_km: float = 0.0
+    w_scale: float = 50.0
+    residual_tol: float = 0.0
+    residual_interval: int = 1
+    global_max_iter: int = 200
+    global_tol: float = 1e-5
+    max_contacts: int = 1 << 20
+    global_refine: int = 1
+
+    def to_c(self) -> AdmmConfigC:
+        c = AdmmConfigC()
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            setattr(c, f.name, int(v) if isinstance(v, bool) else v)
+        return c
+
+
+class Scene:
+    """Mesh + surface topology + elastic constraints + lumped mass (host side)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        info = SceneInfoC()
+        _check(lib().pa_scene_get_info(self._h, C.byref(info)), "scene info")
+        self.info = {k: getattr(info, k) for k, _ in SceneInfoC._fields_}
+
+    @classmethod
+    def generate(cls, kind: int, side: int = 0, seed: int = 0, radius: float = 0.0, drop: float = 0.0) -> "Scene":
+        h = C.c_void_p()
+        _check(lib().pa_scene_generate(kind, side, seed, radius, drop, C.byref(h)), "scene generate")
+        return cls(h)
+
+    @classmethod
+    def from_mesh(cls, x, n_dyn, density, tris=None, tri_k=None, bend_k=None, tets=None, tet_k=None,
+                  tet_boundary=None, pins=None) -> "Scene":
+        x = np.ascontiguousarray(x, np.float64).reshape(-1, 3)
+        n_all = x.shape[0]
+        arr = lambda a, dt, shape: np.ascontiguousarray(np.zeros(shape, dt) if a is None else a, dt)
+        tris = arr(tris, np.int32, (0, 3)).reshape(-1, 3)
+        tets = arr(tets, np.int32, (0, 4)).reshape(-1, 4)
+        tb = arr(tet_boundary, np.int32, (0, 3)).reshape(-1, 3)
+        pins = arr(pins, np.int32, (0,))
+        density = arr(density, np.float64, (n_all,))
+        tri_k = arr(tri_k, np.float64, (tris.shape[0],))
+        tet_k = arr(tet_k, np.float64, (tets.shape[0],))
+        bend_k = arr(bend_k, np.float64, (n_all,))
+        d = MeshDescC(n_all, n_dyn, _ptr(x), _ptr(density), tris.shape[0], _ptr(tris), _ptr(tri_k), _ptr(bend_k),
+                      tets.shape[0], _ptr(tets), _ptr(tet_k), tb.shape[0], _ptr(tb), pins.shape[0], _ptr(pins))
+        h = C.c_void_p()
+        _check(lib().pa_scene_create(C.byref(d), C.byref(h)), "scene create")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_scene_destroy(self._h)
+            self._h = None
+
+    def mesh(self) -> dict:
+        i = self.info
+        out = dict(x=np.zeros((i["n_all"], 3)), density=np.zeros(i["n_all"]), tris=np.zeros((i["n_tri"], 3), np.int32),
+                   tri_k=np.zeros(i["n_tri"]), bend_k=np.zeros(i["n_all"]), tets=np.zeros((i["n_tet"], 4), np.int32),
+                   tet_k=np.zeros(i["n_tet"]), tet_boundary=np.zeros((i["n_tet_boundary"], 3), np.int32),
+                   pins=np.zeros(i["n_pins"], np.int32))
+        _check(lib().pa_scene_get_mesh(self._h, *[_ptr(out[k]) for k in ("x", "density", "tris", "tri_k", "bend_k", "tets",
+                                                                          "tet_k", "tet_boundary", "pins")]), "scene mesh")
+        out["n_dyn"] = i["n_dyn"]
+        return out
+
+    def topology(self) -> dict:
+        i = self.info
+        sv = np.zeros(i["n_surf_verts"], np.int32)
+        st = np.zeros((i["n_surf_tris"], 3), np.int32)
+        se = np.zeros((i["n_surf_edges"], 2), np.int32)
+        _check(lib().pa_scene_get_topology(self._h, _ptr(sv), _ptr(st), _ptr(se)), "scene topology")
+        return dict(surf_verts=sv, surf_tris=st, surf_edges=se)
+
+    def mass(self) -> np.ndarray:
+        m = np.zeros(self.info["n_all"])
+        _check(lib().pa_scene_get_mass(self._h, _ptr(m)), "scene mass")
+        return m
+
+    def constraints(self) -> dict:
+        s = ConstraintSoAC()
+        _check(lib().pa_scene_get_constraints(self._h, C.byref(s)), "scene constraints")
+
+        def view(addr, n, dt):
+            if n == 0 or not addr:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(addr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
+        nb = s.n_bend
+        bend_ptr = view(s.bend_ptr, nb + 1, np.int32) if nb else np.zeros(1, np.int32)
+        ne = int(bend_ptr[-1])
+        return dict(
+            n_tri=s.n_tri, n_tet=s.n_tet, n_bend=nb, n_pin=s.n_pin, tri_row0=s.tri_row0, tet_row0=s.tet_row0,
+            bend_row0=s.bend_row0, pin_row0=s.pin_row0,
+            tri_inds=view(s.tri_inds, 3 * s.n_tri, np.int32).reshape(-1, 3), tri_binv=view(s.tri_binv, 4 * s.n_tri, np.float64).reshape(-1, 2, 2),
+            tri_w=view(s.tri_w, s.n_tri, np.float64), tet_inds=view(s.tet_inds, 4 * s.n_tet, np.int32).reshape(-1, 4),
+            tet_binv=view(s.tet_binv, 9 * s.n_tet, np.float64).reshape(-1, 3, 3), tet_w=view(s.tet_w, s.n_tet, np.float64),
+            bend_ptr=bend_ptr, bend_inds=view(s.bend_inds, ne, np.int32), bend_coef=view(s.bend_coef, ne, np.float64),
+            bend_w=view(s.bend_w, nb, np.float64), pin_inds=view(s.pin_inds, s.n_pin, np.int32),
+            pin_pos=view(s.pin_pos, 3 * s.n_pin, np.float64).reshape(-1, 3), pin_w=view(s.pin_w, s.n_pin, np.float64))
+
+    def surface_flags(self) -> np.ndarray:
+        f = np.zeros(self.info["n_all"], np.uint8)
+        f[self.topology()["surf_verts"]] = 1
+        return f
+
+
+class CollisionDetector:
+    """ICollisionDetector over an LBVH on the GPU (≙ BVH_GPU)."""
+
+    def __init__(self, n_all: int, n_dyn: int, surf_verts, surf_tris, surf_edges, capacity: int, thickness: float):
+        self._keep = [np.ascontiguousarray(surf_verts, np.int32), np.ascontiguousarray(surf_tris, np.int32).reshape(-1),
+                      np.ascontiguousarray(surf_edges, np.int32).reshape(-1)]
+        sv, st, se = self._keep
+        s = SurfaceC(n_all, n_dyn, sv.size, st.size // 3, se.size // 2, _ptr(sv), _ptr(st), _ptr(se))
+        self._h = C.c_void_p()
+        self.n_all = n_all
+        _check(lib().pa_detector_create(C.byref(s), capacity, C.c_double(thickness), C.byref(self._h)), "detector create")
+
+    @classmethod
+    def for_scene(cls, scene: Scene, capacity: int, thickness: float) -> "CollisionDetector":
+        t = scene.topology()
+        return cls(scene.info["n_all"], scene.info["n_dyn"], t["surf_verts"], t["surf_tris"], t["surf_edges"], capacity, thickness)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_detector_destroy(self._h)
+            self._h = None
+
+    def detect(self, pos) -> int:
+        x = np.ascontiguousarray(pos, np.float64).reshape(-1)
+        n = C.c_int32()
+        _check(lib().pa_detector_detect(self._h, _ptr(x), C.byref(n)), "detect")
+        return n.value
+
+    def _counts(self):
+        a, b, c = C.c_int32(), C.c_int32(), C.c_int32()
+        _check(lib().pa_detector_counts(self._h, C.byref(a), C.byref(b), C.byref(c)), "counts")
+        return a.value, b.value, c.value
+
+    def contact_count(self) -> int:
+        return self._counts()[0]
+
+    def max_slots(self) -> int:
+        return self._counts()[1]
+
+    def peak_contact_count(self) -> int:
+        return self._counts()[2]
+
+    def contacts(self) -> tuple[np.ndarray, np.ndarray]:
+        """(ints[n, 7] = type, prim0, prim1, v0..v3; reals[n, 9] = bary[4], normal[3], h_cN, dist)."""
+        n = self.contact_count()
+        ints = np.zeros((n, 7), np.int32)
+        reals = np.zeros((n, 9))
+        got = C.c_int32()
+        _check(lib().pa_detector_get_contacts(self._h, _ptr(ints), _ptr(reals), n, C.byref(got)), "get contacts")
+        return ints, reals
+
+    def points(self) -> np.ndarray:
+        out = np.zeros((self.max_slots(), 3), np.float32)
+        _check(lib().pa_detector_points(self._h, _ptr(out)), "points")
+        return out
+
+    def normals(self) -> np.ndarray:
+        out = np.zeros((self.max_slots(), 3), np.float32)
+        _check(lib().pa_detector_normals(self._h, _ptr(out)), "normals")
+        return out
+
+    def clear(self) -> None:
+        _check(lib().pa_detector_clear(self._h), "clear")
+
+
+class ADMMBackendHIP:
+    """IADMMBackend on one MI355X (≙ ADMMBackendGPU), with the Solver entry points
+    (init / reset / getVertices / getVelocities) the reference reaches through
+    as_solver()."""
+
+    def __init__(self, device: int = 0, verbose: int = 0, _handle: Optional[C.c_void_p] = None):
+        self._h = _handle if _handle is not None else C.c_void_p()
+        if _handle is None:
+            o = DeviceOptsC(device, verbose)
+            _check(lib().pa_backend_create(C.byref(o), C.byref(self._h)), "backend create")
+        self.n_all = self.n_dyn = self.m = 0
+        self._detector = None
+
+    @classmethod
+    def from_scene(cls, scene: Scene, cfg: AdmmConfig, device: int = 0) -> "ADMMBackendHIP":
+        h = C.c_void_p()
+        c = cfg.to_c()
+        o = DeviceOptsC(device, 0)
+        _check(lib().pa_backend_create_from_scene(scene._h, C.byref(c), C.byref(o), C.byref(h)), "backend from scene")
+        b = cls(_handle=h)
+        b.n_all, b.n_dyn, b.m = scene.info["n_all"], scene.info["n_dyn"], scene.info["m_rows"]
+        return b
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_backend_destroy(self._h)
+            self._h = None
+
+    def _err(self, code, what):
+        if code != PA_OK:
+            raise PAError(code, f"{what}: {lib().pa_backend_last_error(self._h).decode(errors='replace')}")
+
+    # --- IADMMBackend ---
+    def apply_config(self, cfg: AdmmConfig) -> None:
+        c = cfg.to_c()
+        self._err(lib().pa_backend_apply_config(self._h, C.byref(c)), "apply_config")
+
+    def set_constraints(self, scene: Scene) -> None:
+        s = ConstraintSoAC()
+        _check(lib().pa_scene_get_constraints(scene._h, C.byref(s)))
+        self._err(lib().pa_backend_set_constraints(self._h, C.byref(s)), "set_constraints")
+
+    def set_constraint_dim(self, m: int) -> None:
+        self.m = m
+        self._err(lib().pa_backend_set_constraint_dim(self._h, m), "set_constraint_dim")
+
+    def finalize_constraints(self) -> None:
+        self._err(lib().pa_backend_finalize_constraints(self._h), "finalize_constraints")
+
+    def set_sync_to_host(self, enabled: bool) -> None:
+        self._err(lib().pa_backend_set_sync_to_host(self._h, int(enabled)), "set_sync_to_host")
+
+    def step(self) -> dict:
+        st = StepStatsC()
+        code = lib().pa_backend_step(self._h, C.byref(st))
+        if code != PA_OK:
+            self._err(code, "step")
+        return {k: getattr(st, k) for k, _ in StepStatsC._fields_}
+
+    # --- Solver entry points ---
+    def init(self, verts, mass, n_dyn: int, surface_flags, pins) -> None:
+        x = np.ascontiguousarray(verts, np.float64).reshape(-1)
+        m = np.ascontiguousarray(mass, np.float64)
+        sf = np.ascontiguousarray(surface_flags, np.uint8)
+        p = np.ascontiguousarray(pins, np.int32)
+        self.n_all, self.n_dyn = m.size, n_dyn
+        self._err(lib().pa_backend_init(self._h, _ptr(x), _ptr(m), m.size, n_dyn, _ptr(sf), p.size, _ptr(p)), "init")
+
+    def attach_detector(self, det: CollisionDetector) -> None:
+        self._detector = det
+        self._err(lib().pa_backend_attach_detector(self._h, det._h), "attach_detector")
+
+    def reset(self, ini_verts) -> None:
+        x = np.ascontiguousarray(ini_verts, np.float64).reshape(-1)
+        self._err(lib().pa_backend_reset(self._h, _ptr(x)), "reset")
+
+    def get_state(self) -> tuple[np.ndarray, np.ndarray]:
+        x = np.zeros((self.n_all, 3))
+        v = np.zeros((self.n_all, 3))
+        self._err(lib().pa_backend_get_state(self._h, _ptr(x), _ptr(v)), "get_state")
+        return x, v
+
+    def getVertices(self) -> np.ndarray:
+        return self.get_state()[0]
+
+    def getVelocities(self) -> np.ndarray:
+        return self.get_state()[1]
+
+    def set_state(self, x=None, v=None) -> None:
+        xa = None if x is None else np.ascontiguousarray(x, np.float64).reshape(-1)
+        va = None if v is None else np.ascontiguousarray(v, np.float64).reshape(-1)
+        self._err(lib().pa_backend_set_state(self._h, _ptr(xa), _ptr(va)), "set_state")
+
+    def get_duals(self) -> tuple[np.ndarray, np.ndarray]:
+        ue = np.zeros((self.m, 3))
+        uc = np.zeros((self.n_dyn, 3))
+        self._err(lib().pa_backend_get_duals(self._h, _ptr(ue), _ptr(uc)), "get_duals")
+        return ue, uc
+
+    def set_duals(self, ue=None, uc=None) -> None:
+        a = None if ue is None else np.ascontiguousarray(ue, np.float64).reshape(-1)
+        b = None if uc is None else np.ascontiguousarray(uc, np.float64).reshape(-1)
+        self._err(lib().pa_backend_set_duals(self._h, _ptr(a), _ptr(b)), "set_duals")
+
+    def set_static_state(self, x_static=None, v_static=None) -> None:
+        a = None if x_static is None else np.ascontiguousarray(x_static, np.float64).reshape(-1)
+        b = None if v_static is None else np.ascontiguousarray(v_static, np.float64).reshape(-1)
+        self._err(lib().pa_backend_set_static_state(self._h, _ptr(a), _ptr(b)), "set_static_state")
+
+    # --- sub-step entry points (parity tests, profiling) ---
+    def project_elastic(self, x, ue) -> np.ndarray:
+        xa = np.ascontiguousarray(x, np.float64).reshape(-1)
+        ua = np.ascontiguousarray(ue, np.float64).reshape(-1)
+        z = np.zeros((self.m, 3))
+        self._err(lib().pa_backend_project_elastic(self._h, _ptr(xa), _ptr(ua), _ptr(z)), "project_elastic")
+        return z
+
+    def project_contacts(self, p, n_contacts: int) -> tuple[np.ndarray, np.ndarray]:
+        pa_ = np.ascontiguousarray(p, np.float64).reshape(-1).copy()
+        rc = np.zeros((max(n_contacts, 1), 3))
+        self._err(lib().pa_backend_project_contacts(self._h, _ptr(pa_), _ptr(rc)), "project_contacts")
+        return pa_.reshape(-1, 3), rc[:n_contacts]
+
+    def global_solve(self, rhs, x_init) -> tuple[np.ndarray, int]:
+        b = np.ascontiguousarray(rhs, np.float64).reshape(-1)
+        x0 = np.ascontiguousarray(x_init, np.float64).reshape(-1)
+        x = np.zeros((self.n_dyn, 3))
+        it = C.c_int32()
+        self._err(lib().pa_backend_global_solve(self._h, _ptr(b), _ptr(x0), _ptr(x), C.byref(it)), "global_solve")
+        return x, it.value
+
+    def set_profiling(self, mode: int) -> None:
+        """mode bit 0: time each PCG SpMV launch; bit 1: time the step phases."""
+        self._err(lib().pa_backend_set_profiling(self._h, int(mode)), "set_profiling")
+
+    def sizes(self) -> dict:
+        o = np.zeros(7, np.int64)
+        self._err(lib().pa_backend_get_sizes(self._h, _ptr(o)), "get_sizes")
+        return dict(zip(("n_all", "n_dyn", "m_rows", "nnz_a_off", "nnz_damp", "nnz_dt", "n_contacts"), (int(v) for v in o)))
+
+    def phase_times(self) -> np.ndarray:
+        out = np.zeros(6)
+        self._err(lib().pa_backend_phase_times(self._h, _ptr(out)), "phase_times")
+        return out
+
+    def kernel_time(self, reset: bool = False) -> tuple[float, int]:
+        avg = C.c_double()
+        n = C.c_int64()
+        self._err(lib().pa_backend_kernel_time(self._h, 0, int(reset), C.byref(avg), C.byref(n)), "kernel_time")
+        return avg.value, n.value
+
+
+def gpu_available() -> bool:
+    """True when a HIP device is visible to the library (no allocation)."""
+    try:
+        import torch  # device enumeration only
+        return torch.cuda.is_available() and torch.cuda.device_count() > 0
+    except Exception:
+        return False

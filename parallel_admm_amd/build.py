@@ -1,0 +1,73 @@
+"""Builds libpa_admm.so (HIP, gfx950) in-tree: parallel_admm_amd/_lib/.
+
+Every HIP translation unit is compiled with packed FP32 instructions disabled
+(``-Xclang -target-feature -Xclang -packed-fp32-ops``); the detector unit is
+also compiled with ``-ffp-contract=off`` so its FP64 predicate has IEEE
+semantics.  Run ``python -m parallel_admm_amd.build``.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT_DIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(OUT_DIR, "libpa_admm.so")
+ARCH = os.environ.get("PA_OFFLOAD_ARCH", "gfx950")
+
+SOURCES = [
+    ("pa_scene.cpp", False, []),
+    ("pa_capi.cpp", True, []),
+    ("pa_prims.hip", True, []),
+    ("pa_lbvh.hip", True, ["-ffp-contract=off"]),
+    ("pa_solver.hip", True, []),
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
+            return cand
+    return "hipcc"
+
+
+def _compile(src: str, is_hip: bool, extra: list[str], verbose: bool) -> str:
+    obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+    cmd = [_hipcc(), "-c", "-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+           f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}", "-o", obj]
+    if is_hip:
+        cmd += ["-x", "hip", f"--offload-arch={ARCH}",
+                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+    cmd += extra + [os.path.join(CSRC, src)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {src}\n{res.stdout}\n{res.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    """Compiles every source and links libpa_admm.so; returns its path."""
+    os.makedirs(OUT_DIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), os.cpu_count() or 4)) as ex:
+        objs = list(ex.map(lambda s: _compile(s[0], s[1], s[2], verbose), SOURCES))
+    tmp = LIB + ".tmp"
+    cmd = [_hipcc(), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,129 @@
+// ADMM backend state on one gfx950 device (replaces ADMMImplGPU,
+// src/solver/backend_gpu/admm_impl_gpu.{h,cu}, and the kernels of
+// src/solver/{jacobi_solver,contact_solver,compact_sparse_matrix_util}.cu
+// and src/constraint/*_device.cu).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "pa_common.hpp"
+#include "pa_detector.hpp"
+#include "pa_prims.hpp"
+
+namespace pa {
+
+struct HostConstraints {
+    int n_tri = 0, n_tet = 0, n_bend = 0, n_pin = 0;
+    int tri_row0 = 0, tet_row0 = 0, bend_row0 = 0, pin_row0 = 0;
+    std::vector<int32_t> tri_inds, tet_inds, bend_ptr, bend_inds, pin_inds;
+    std::vector<double> tri_binv, tri_w, tet_binv, tet_w, bend_coef, bend_w, pin_pos, pin_w;
+    int rows() const { return 2 * n_tri + 3 * n_tet + n_bend + n_pin; }
+};
+
+struct PhaseTimer {
+    bool enabled = false;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+};
+
+struct Backend {
+    int device = 0;
+    int verbose = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+
+    pa_admm_config cfg{};
+    bool have_cfg = false;
+    HostConstraints hc;
+    bool have_constraints = false;
+    int m_declared = -1;
+    bool finalized = false;
+    bool initialized = false;
+    bool sync_to_host = true;
+
+    int n_all = 0, n = 0, m = 0;
+    std::vector<double> mass_h;
+    std::vector<uint8_t> surface_h, pinned_h;
+    std::vector<double> x_host, v_host; // host mirror (sync_to_host)
+
+    // state
+    DBuf<float> x, x0, v, p, Ue, Uc, z, zprev;
+    DBuf<double> b, bconst; // right-hand side in FP64
+    DBuf<float> mass, wc, wc_inv, wch2; // per vertex (wc* over n_all)
+    DBuf<uint8_t> pinned;
+    // constraints
+    DBuf<int4> tri_v, tet_v;
+    DBuf<float4> tri_binv, pin_pos;
+    DBuf<float> tet_binv, tri_w, tet_w, bend_w, pin_w, bend_coef;
+    DBuf<int32_t> bend_ptr, bend_idx, pin_v;
+    // D^T We^2 h^2 gather (per dynamic vertex)
+    DBuf<int32_t> dt_ptr, dt_row;
+    DBuf<float> dt_coef;
+    // system matrix (off-diagonal CSR + diagonal) and damping matrix (full CSR)
+    DBuf<int32_t> a_ptr, a_col, dm_ptr, dm_col;
+    DBuf<float> a_val, a_diag, a_dinv, dm_val;
+    DBuf<float> a_val_lo, a_diag_lo; // A = hi + lo to ~2^-48 for the FP64 residual
+    int64_t nnz_a_off = 0, nnz_dm = 0;
+    // PCG
+    DBuf<float> cg_r, cg_z, cg_p, cg_q, cg_d;
+    DBuf<double> cg_part;  // per block partial sums
+    DBuf<double> cg_state; // two slots of (rz[3], done[3]); [16..18] reference norm^2, [19] iterations, [20..22] |b|^2
+    int cg_blocks = 0;
+    // contacts
+    Detector* det = nullptr;
+    std::unique_ptr<Detector> own_det;
+    int nc = 0;
+    DBuf<float4> gam, kc, rc, du;
+    DBuf<uint32_t> vcnt, vptr, vfill, vent;
+    PrimScratch scratch;
+    // residual partial sums
+    DBuf<double> res_part, res_sum;
+    int res_blocks = 0;
+    // timing
+    int profiling = 0; // bit 0: PCG SpMV launch events, bit 1: phase events
+    double phase_ms[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_marks;
+    size_t ev_next = 0;
+    double spmv_ms_sum = 0;
+    int64_t spmv_launches = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spmv_marks;
+
+    explicit Backend(const pa_device_opts* o);
+    ~Backend();
+
+    void apply_config(const pa_admm_config& c);
+    void set_constraints(const pa_constraint_soa& c);
+    void set_constraint_dim(int m_rows) { m_declared = m_rows; }
+    void finalize_constraints();
+    void init(const double* x0, const double* mass, int n_all, int n_dyn, const uint8_t* surface, int n_pins, const int32_t* pins);
+    void attach_detector(Detector* d);
+    void step(pa_step_stats* st);
+    void step_impl(pa_step_stats* st);
+    void reset(const double* x0);
+    void get_state(double* x, double* v);
+    void set_state(const double* x, const double* v);
+    void get_duals(double* ue, double* uc);
+    void set_duals(const double* ue, const double* uc);
+    void set_static_state(const double* xs, const double* vs);
+
+    // sub-phases (also used by step)
+    void local_step(bool first_iter);
+    void contact_setup(float eps, float gamma, float hinv);
+    void contact_project(bool first_iter);
+    void contact_sweep_once();
+    void rhs(bool want_residual, bool first_iter);
+    int global_solve(int max_iter, double tol);
+    void dual_update(bool want_residual);
+    void residual_read(double* rp, double* rd);
+
+  private:
+    void assemble_and_upload(const double* x0);
+    hipEvent_t next_event();
+    void mark_begin(int phase, hipEvent_t* e0);
+    void mark_end(int phase, hipEvent_t e0);
+    void collect_phase_times();
+};
+
+} // namespace pa

@@ -1,0 +1,531 @@
+// C ABI entry points (include/pa_admm.h).  Every function converts internal
+// exceptions into a pa_status; nothing here exits the process.
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "pa_admm.h"
+#include "pa_backend.hpp"
+#include "pa_detector.hpp"
+#include "pa_scene.hpp"
+
+struct pa_scene {
+    pa::Scene s;
+};
+struct pa_detector {
+    std::unique_ptr<pa::Detector> d;
+    std::string err;
+};
+struct pa_backend {
+    std::unique_ptr<pa::Backend> b;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <class F> pa_status guard(std::string* err, F&& f) {
+    try {
+        f();
+        return PA_OK;
+    } catch (const pa::Error& e) {
+        if (err) *err = e.what();
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        if (err) *err = "out of host memory";
+        g_last_error = "out of host memory";
+        return PA_ERR_INVALID;
+    } catch (const std::exception& e) {
+        if (err) *err = e.what();
+        g_last_error = e.what();
+        return PA_ERR_GEOMETRY;
+    }
+}
+
+std::vector<uint8_t> surface_flags(const pa::Scene& s) {
+    std::vector<uint8_t> f(s.mesh.n_all, 0);
+    for (int v : s.topo.surf_verts) f[v] = 1;
+    return f;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* pa_last_error_message(void) { return g_last_error.c_str(); }
+
+void pa_admm_config_default(pa_admm_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof(*c));
+    c->dt = 1.0 / 60.0;
+    c->g = 0.98; // ADMMSolverConfig::g default (admm_config.h:27)
+    c->admm_max_iter = 25;
+    c->dcd_interval = 5;
+    c->gs_max_iter = 10;
+    c->warmstart_ue = 1;
+    c->warmstart_uc = 1;
+    c->enable_contact = 1;
+    c->mu = 0.5;
+    c->kappa = 1e6;
+    c->beta = 1e5;
+    c->thickness = 5e-2; // ContactParameter::thickness default (contact_info.cpp:3)
+    c->damp_kl = 1e-4;
+    c->damp_km = 0.0;
+    c->w_scale = 50.0;
+    c->residual_tol = 0.0;
+    c->residual_interval = 1;
+    c->global_max_iter = 200;
+    c->global_tol = 1e-5;
+    c->max_contacts = 1 << 20;
+    c->global_refine = 1;
+}
+
+// ---------------------------------------------------------------- scene
+pa_status pa_scene_create(const pa_mesh_desc* m, pa_scene** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(m && out, PA_ERR_INVALID, "null argument");
+        PA_CHECK(m->n_all > 0 && m->n_dyn > 0 && m->n_dyn <= m->n_all, PA_ERR_INVALID, "bad vertex counts");
+        PA_CHECK(m->x && m->density && m->bend_k, PA_ERR_INVALID, "missing vertex arrays");
+        auto sc = std::make_unique<pa_scene>();
+        pa::SceneMesh& M = sc->s.mesh;
+        M.n_all = m->n_all;
+        M.n_dyn = m->n_dyn;
+        M.x.assign(m->x, m->x + 3 * (size_t)m->n_all);
+        M.density.assign(m->density, m->density + m->n_all);
+        M.bend_k.assign(m->bend_k, m->bend_k + m->n_all);
+        PA_CHECK(m->n_tri == 0 || (m->tris && m->tri_k), PA_ERR_INVALID, "missing triangle arrays");
+        M.tris.assign(m->tris, m->tris + 3 * (size_t)m->n_tri);
+        M.tri_k.assign(m->tri_k, m->tri_k + m->n_tri);
+        PA_CHECK(m->n_tet == 0 || (m->tets && m->tet_k), PA_ERR_INVALID, "missing tet arrays");
+        M.tets.assign(m->tets, m->tets + 4 * (size_t)m->n_tet);
+        M.tet_k.assign(m->tet_k, m->tet_k + m->n_tet);
+        PA_CHECK(m->n_tet_boundary == 0 || m->tet_boundary, PA_ERR_INVALID, "missing tet boundary");
+        M.tet_boundary.assign(m->tet_boundary, m->tet_boundary + 3 * (size_t)m->n_tet_boundary);
+        PA_CHECK(m->n_pins == 0 || m->pins, PA_ERR_INVALID, "missing pins");
+        M.pins.assign(m->pins, m->pins + m->n_pins);
+        for (int32_t v : M.tris) PA_CHECK(v >= 0 && v < M.n_all, PA_ERR_INVALID, "triangle index out of range");
+        for (int32_t v : M.tets) PA_CHECK(v >= 0 && v < M.n_dyn, PA_ERR_INVALID, "tet index out of range");
+        for (int32_t v : M.tet_boundary) PA_CHECK(v >= 0 && v < M.n_all, PA_ERR_INVALID, "boundary index out of range");
+        try {
+            pa::scene_finalize(sc->s);
+        } catch (const std::runtime_error& e) {
+            throw pa::Error(PA_ERR_GEOMETRY, e.what());
+        }
+        *out = sc.release();
+    });
+}
+
+pa_status pa_scene_generate(int32_t kind, int32_t side, uint64_t seed, double radius, double drop, pa_scene** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(out, PA_ERR_INVALID, "null argument");
+        PA_CHECK(kind >= 1 && kind <= 4, PA_ERR_INVALID, "unknown scene kind");
+        PA_CHECK(side == 0 || (side >= 2 && side <= 4096), PA_ERR_INVALID, "side out of range");
+        auto sc = std::make_unique<pa_scene>();
+        try {
+            sc->s = pa::scene_generate(kind, side, seed, radius, drop);
+        } catch (const std::runtime_error& e) {
+            throw pa::Error(PA_ERR_GEOMETRY, e.what());
+        }
+        *out = sc.release();
+    });
+}
+
+void pa_scene_destroy(pa_scene* s) { delete s; }
+
+pa_status pa_scene_get_info(const pa_scene* sc, pa_scene_info* o) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc && o, PA_ERR_INVALID, "null argument");
+        const pa::Scene& s = sc->s;
+        std::memset(o, 0, sizeof(*o));
+        o->n_all = s.mesh.n_all;
+        o->n_dyn = s.mesh.n_dyn;
+        o->m_rows = s.cons.m_rows;
+        o->n_tri = (int32_t)(s.mesh.tris.size() / 3);
+        o->n_tet = (int32_t)(s.mesh.tets.size() / 4);
+        o->n_tet_boundary = (int32_t)(s.mesh.tet_boundary.size() / 3);
+        o->n_pins = (int32_t)s.mesh.pins.size();
+        o->n_tri_c = (int32_t)s.cons.tri_w.size();
+        o->n_tet_c = (int32_t)s.cons.tet_w.size();
+        o->n_bend_c = (int32_t)s.cons.bend_w.size();
+        o->n_pin_c = (int32_t)s.cons.pin_w.size();
+        o->n_surf_verts = (int32_t)s.topo.surf_verts.size();
+        o->n_surf_tris = (int32_t)(s.topo.surf_tris.size() / 3);
+        o->n_surf_edges = (int32_t)(s.topo.surf_edges.size() / 2);
+        o->bend_entries = (int64_t)s.cons.bend_inds.size();
+        o->nnz_d = 6 * (int64_t)o->n_tri_c + 12 * (int64_t)o->n_tet_c + o->bend_entries + o->n_pin_c;
+    });
+}
+
+pa_status pa_scene_get_mesh(const pa_scene* sc, double* x, double* density, int32_t* tris, double* tri_k, double* bend_k,
+                            int32_t* tets, double* tet_k, int32_t* tet_boundary, int32_t* pins) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc, PA_ERR_INVALID, "null scene");
+        const pa::SceneMesh& M = sc->s.mesh;
+        auto cp = [](auto* dst, const auto& v) {
+            if (dst) std::copy(v.begin(), v.end(), dst);
+        };
+        cp(x, M.x);
+        cp(density, M.density);
+        cp(tris, M.tris);
+        cp(tri_k, M.tri_k);
+        cp(bend_k, M.bend_k);
+        cp(tets, M.tets);
+        cp(tet_k, M.tet_k);
+        cp(tet_boundary, M.tet_boundary);
+        cp(pins, M.pins);
+    });
+}
+
+pa_status pa_scene_get_topology(const pa_scene* sc, int32_t* sv, int32_t* st, int32_t* se) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc, PA_ERR_INVALID, "null scene");
+        const pa::SceneTopology& T = sc->s.topo;
+        if (sv) std::copy(T.surf_verts.begin(), T.surf_verts.end(), sv);
+        if (st) std::copy(T.surf_tris.begin(), T.surf_tris.end(), st);
+        if (se) std::copy(T.surf_edges.begin(), T.surf_edges.end(), se);
+    });
+}
+
+pa_status pa_scene_get_mass(const pa_scene* sc, double* mass) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc && mass, PA_ERR_INVALID, "null argument");
+        std::copy(sc->s.mass.begin(), sc->s.mass.end(), mass);
+    });
+}
+
+pa_status pa_scene_get_constraints(const pa_scene* sc, pa_constraint_soa* o) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc && o, PA_ERR_INVALID, "null argument");
+        const pa::SceneConstraints& C = sc->s.cons;
+        std::memset(o, 0, sizeof(*o));
+        o->n_tri = (int32_t)C.tri_w.size();
+        o->n_tet = (int32_t)C.tet_w.size();
+        o->n_bend = (int32_t)C.bend_w.size();
+        o->n_pin = (int32_t)C.pin_w.size();
+        o->tri_row0 = C.tri_row0;
+        o->tet_row0 = C.tet_row0;
+        o->bend_row0 = C.bend_row0;
+        o->pin_row0 = C.pin_row0;
+        o->tri_inds = C.tri_inds.data();
+        o->tri_binv = C.tri_binv.data();
+        o->tri_w = C.tri_w.data();
+        o->tet_inds = C.tet_inds.data();
+        o->tet_binv = C.tet_binv.data();
+        o->tet_w = C.tet_w.data();
+        o->bend_ptr = C.bend_ptr.data();
+        o->bend_inds = C.bend_inds.data();
+        o->bend_coef = C.bend_coef.data();
+        o->bend_w = C.bend_w.data();
+        o->pin_inds = C.pin_inds.data();
+        o->pin_pos = C.pin_pos.data();
+        o->pin_w = C.pin_w.data();
+    });
+}
+
+// ---------------------------------------------------------------- detector
+pa_status pa_detector_create(const pa_surface* s, int32_t capacity, double thickness, pa_detector** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(s && out, PA_ERR_INVALID, "null argument");
+        PA_CHECK(s->n_all > 0 && s->n_dyn >= 0 && s->n_dyn <= s->n_all, PA_ERR_INVALID, "bad vertex counts");
+        PA_CHECK(s->n_surf_verts >= 0 && s->n_surf_tris >= 0 && s->n_surf_edges >= 0, PA_ERR_INVALID, "bad surface counts");
+        int count = 0;
+        PA_CHECK(hipGetDeviceCount(&count) == hipSuccess && count > 0, PA_ERR_NO_DEVICE, "no HIP device visible");
+        auto d = std::make_unique<pa_detector>();
+        d->d = std::make_unique<pa::Detector>(s->n_all, s->n_dyn, s->n_surf_verts, s->surf_verts, s->n_surf_tris, s->surf_tris,
+                                              s->n_surf_edges, s->surf_edges, capacity, thickness, nullptr);
+        *out = d.release();
+    });
+}
+
+void pa_detector_destroy(pa_detector* d) { delete d; }
+
+pa_status pa_detector_detect(pa_detector* d, const double* xh, int32_t* n_contacts) {
+    return guard(d ? &d->err : nullptr, [&] {
+        PA_CHECK(d && xh, PA_ERR_INVALID, "null argument");
+        pa::Detector& D = *d->d;
+        std::vector<float> xf(3 * (size_t)D.n_all);
+        for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)xh[i];
+        D.x_own.reserve(xf.size());
+        D.x_own.upload(xf.data(), xf.size(), D.stream);
+        int nc = D.detect(D.x_own.p);
+        if (n_contacts) *n_contacts = nc;
+    });
+}
+
+pa_status pa_detector_get_contacts(pa_detector* d, int32_t* ints, double* reals, int32_t cap, int32_t* n_contacts) {
+    return guard(d ? &d->err : nullptr, [&] {
+        PA_CHECK(d, PA_ERR_INVALID, "null detector");
+        pa::Detector& D = *d->d;
+        const int nc = D.n_contacts;
+        if (n_contacts) *n_contacts = nc;
+        const int k = std::min(nc, (int)cap);
+        if (k <= 0) return;
+        std::vector<int32_t> ty(k), p0(k), p1(k);
+        std::vector<int4> vv(k);
+        std::vector<float4> bb(k), nn(k);
+        std::vector<double> dd(k);
+        D.c.type.download(ty.data(), k, D.stream);
+        D.c.prim0.download(p0.data(), k, D.stream);
+        D.c.prim1.download(p1.data(), k, D.stream);
+        D.c.v.download(vv.data(), k, D.stream);
+        D.c.bary.download(bb.data(), k, D.stream);
+        D.c.nrm.download(nn.data(), k, D.stream);
+        D.c.dist.download(dd.data(), k, D.stream);
+        PA_HIP(hipStreamSynchronize(D.stream));
+        for (int i = 0; i < k; i++) {
+            if (ints) {
+                int32_t* o = ints + 7 * (size_t)i;
+                o[0] = ty[i];
+                o[1] = p0[i];
+                o[2] = p1[i];
+                o[3] = vv[i].x;
+                o[4] = vv[i].y;
+                o[5] = vv[i].z;
+                o[6] = vv[i].w;
+            }
+            if (reals) {
+                double* o = reals + 9 * (size_t)i;
+                o[0] = bb[i].x;
+                o[1] = bb[i].y;
+                o[2] = bb[i].z;
+                o[3] = bb[i].w;
+                o[4] = nn[i].x;
+                o[5] = nn[i].y;
+                o[6] = nn[i].z;
+                o[7] = nn[i].w;
+                o[8] = dd[i];
+            }
+        }
+    });
+}
+
+pa_status pa_detector_counts(const pa_detector* d, int32_t* count, int32_t* max_slots, int32_t* peak) {
+    return guard(nullptr, [&] {
+        PA_CHECK(d, PA_ERR_INVALID, "null detector");
+        if (count) *count = d->d->n_contacts;
+        if (max_slots) *max_slots = d->d->c.cap;
+        if (peak) *peak = d->d->peak;
+    });
+}
+
+static pa_status detector_field(pa_detector* d, float* out, bool points) {
+    return guard(d ? &d->err : nullptr, [&] {
+        PA_CHECK(d && out, PA_ERR_INVALID, "null argument");
+        pa::Detector& D = *d->d;
+        const int cap = D.c.cap, nc = D.n_contacts;
+        std::fill(out, out + 3 * (size_t)cap, 0.f);
+        if (nc == 0) return;
+        std::vector<float4> nn(nc);
+        if (!points) {
+            D.c.nrm.download(nn.data(), nc, D.stream);
+            PA_HIP(hipStreamSynchronize(D.stream));
+            for (int i = 0; i < nc; i++) {
+                out[3 * i] = nn[i].x;
+                out[3 * i + 1] = nn[i].y;
+                out[3 * i + 2] = nn[i].z;
+            }
+            return;
+        }
+        // contact point: average of the two closest points, from the stored
+        // weights (c0 + c1)/2 with c0 - c1 = sum bary_k x_k
+        std::vector<int4> vv(nc);
+        std::vector<float4> bb(nc);
+        std::vector<int32_t> ty(nc);
+        std::vector<float> xf(3 * (size_t)D.n_all);
+        D.c.v.download(vv.data(), nc, D.stream);
+        D.c.bary.download(bb.data(), nc, D.stream);
+        D.c.type.download(ty.data(), nc, D.stream);
+        PA_CHECK(D.x_own.p, PA_ERR_INVALID, "points() needs positions from detect()");
+        D.x_own.download(xf.data(), xf.size(), D.stream);
+        PA_HIP(hipStreamSynchronize(D.stream));
+        for (int i = 0; i < nc; i++) {
+            int v[4] = {vv[i].x, vv[i].y, vv[i].z, vv[i].w};
+            float w[4] = {bb[i].x, bb[i].y, bb[i].z, bb[i].w};
+            for (int a = 0; a < 3; a++) {
+                float c0, c1;
+                if (ty[i] == 0) {
+                    c0 = xf[3 * v[0] + a];
+                    c1 = -(w[1] * xf[3 * v[1] + a] + w[2] * xf[3 * v[2] + a] + w[3] * xf[3 * v[3] + a]);
+                } else {
+                    c0 = w[0] * xf[3 * v[0] + a] + w[1] * xf[3 * v[1] + a];
+                    c1 = -(w[2] * xf[3 * v[2] + a] + w[3] * xf[3 * v[3] + a]);
+                }
+                out[3 * i + a] = 0.5f * (c0 + c1);
+            }
+        }
+    });
+}
+
+pa_status pa_detector_points(pa_detector* d, float* p) { return detector_field(d, p, true); }
+pa_status pa_detector_normals(pa_detector* d, float* n) { return detector_field(d, n, false); }
+
+pa_status pa_detector_clear(pa_detector* d) {
+    return guard(nullptr, [&] {
+        PA_CHECK(d, PA_ERR_INVALID, "null detector");
+        d->d->n_contacts = 0;
+    });
+}
+
+// ---------------------------------------------------------------- backend
+pa_status pa_backend_create(const pa_device_opts* o, pa_backend** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(out, PA_ERR_INVALID, "null argument");
+        auto b = std::make_unique<pa_backend>();
+        b->b = std::make_unique<pa::Backend>(o);
+        *out = b.release();
+    });
+}
+
+void pa_backend_destroy(pa_backend* b) { delete b; }
+
+const char* pa_backend_last_error(const pa_backend* b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+#define BK(...) return guard(b ? &b->err : nullptr, [&] { PA_CHECK(b, PA_ERR_INVALID, "null backend"); __VA_ARGS__; })
+
+pa_status pa_backend_apply_config(pa_backend* b, const pa_admm_config* c) {
+    BK(PA_CHECK(c, PA_ERR_INVALID, "null config"); b->b->apply_config(*c));
+}
+pa_status pa_backend_set_constraints(pa_backend* b, const pa_constraint_soa* c) {
+    BK(PA_CHECK(c, PA_ERR_INVALID, "null constraints"); b->b->set_constraints(*c));
+}
+pa_status pa_backend_set_constraint_dim(pa_backend* b, int32_t m) {
+    BK(PA_CHECK(m >= 0, PA_ERR_INVALID, "negative dimension"); b->b->set_constraint_dim(m));
+}
+pa_status pa_backend_finalize_constraints(pa_backend* b) { BK(b->b->finalize_constraints()); }
+pa_status pa_backend_init(pa_backend* b, const double* x0, const double* mass, int32_t n_all, int32_t n_dyn, const uint8_t* surf,
+                          int32_t n_pins, const int32_t* pins) {
+    BK(PA_CHECK(x0 && mass, PA_ERR_INVALID, "null state"); PA_CHECK(n_pins == 0 || pins, PA_ERR_INVALID, "null pins");
+       b->b->init(x0, mass, n_all, n_dyn, surf, n_pins, pins));
+}
+pa_status pa_backend_set_sync_to_host(pa_backend* b, int32_t on) { BK(b->b->sync_to_host = on != 0); }
+pa_status pa_backend_attach_detector(pa_backend* b, pa_detector* d) {
+    BK(PA_CHECK(d, PA_ERR_INVALID, "null detector"); b->b->attach_detector(d->d.get()));
+}
+pa_status pa_backend_step(pa_backend* b, pa_step_stats* st) { BK(b->b->step(st)); }
+pa_status pa_backend_reset(pa_backend* b, const double* x0) { BK(PA_CHECK(x0, PA_ERR_INVALID, "null state"); b->b->reset(x0)); }
+pa_status pa_backend_get_state(pa_backend* b, double* x, double* v) { BK(b->b->get_state(x, v)); }
+pa_status pa_backend_set_state(pa_backend* b, const double* x, const double* v) { BK(b->b->set_state(x, v)); }
+pa_status pa_backend_get_duals(pa_backend* b, double* ue, double* uc) { BK(b->b->get_duals(ue, uc)); }
+pa_status pa_backend_set_duals(pa_backend* b, const double* ue, const double* uc) { BK(b->b->set_duals(ue, uc)); }
+pa_status pa_backend_set_static_state(pa_backend* b, const double* xs, const double* vs) { BK(b->b->set_static_state(xs, vs)); }
+pa_status pa_backend_set_profiling(pa_backend* b, int32_t mode) { BK(b->b->profiling = mode & 3); }
+
+pa_status pa_backend_create_from_scene(const pa_scene* sc, const pa_admm_config* cfg, const pa_device_opts* o, pa_backend** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc && cfg && out, PA_ERR_INVALID, "null argument");
+        const pa::Scene& s = sc->s;
+        auto bb = std::make_unique<pa_backend>();
+        bb->b = std::make_unique<pa::Backend>(o);
+        pa::Backend& B = *bb->b;
+        B.apply_config(*cfg);
+        pa_constraint_soa soa;
+        PA_CHECK(pa_scene_get_constraints(sc, &soa) == PA_OK, PA_ERR_INVALID, "scene constraints");
+        B.set_constraints(soa);
+        B.set_constraint_dim(s.cons.m_rows);
+        B.finalize_constraints();
+        std::vector<uint8_t> surf = surface_flags(s);
+        B.init(s.mesh.x.data(), s.mass.data(), s.mesh.n_all, s.mesh.n_dyn, surf.data(), (int)s.mesh.pins.size(), s.mesh.pins.data());
+        if (cfg->enable_contact) {
+            const auto& T = s.topo;
+            B.own_det = std::make_unique<pa::Detector>(s.mesh.n_all, s.mesh.n_dyn, (int)T.surf_verts.size(), T.surf_verts.data(),
+                                                       (int)(T.surf_tris.size() / 3), T.surf_tris.data(), (int)(T.surf_edges.size() / 2),
+                                                       T.surf_edges.data(), cfg->max_contacts, cfg->thickness, B.stream);
+            B.attach_detector(B.own_det.get());
+        }
+        *out = bb.release();
+    });
+}
+
+pa_status pa_backend_project_elastic(pa_backend* b, const double* x, const double* ue, double* zo) {
+    BK({
+        pa::Backend& B = *b->b;
+        PA_CHECK(B.initialized, PA_ERR_INVALID, "init first");
+        PA_CHECK(x && ue && zo, PA_ERR_INVALID, "null argument");
+        std::vector<float> xf(x, x + 3 * (size_t)B.n_all), uf(ue, ue + 3 * (size_t)B.m), zf(3 * (size_t)B.m);
+        B.x.upload(xf.data(), xf.size(), B.stream);
+        B.Ue.upload(uf.data(), uf.size(), B.stream);
+        B.local_step(true);
+        B.z.download(zf.data(), zf.size(), B.stream);
+        PA_HIP(hipStreamSynchronize(B.stream));
+        for (size_t i = 0; i < zf.size(); i++) zo[i] = zf[i];
+    });
+}
+
+pa_status pa_backend_project_contacts(pa_backend* b, double* p, double* rco) {
+    BK({
+        pa::Backend& B = *b->b;
+        PA_CHECK(B.initialized && B.det, PA_ERR_INVALID, "init and attach a detector first");
+        PA_CHECK(p, PA_ERR_INVALID, "null p");
+        const double hd = B.cfg.dt;
+        const float eps = (float)(1.0 / (hd * hd * B.cfg.kappa + hd * B.cfg.beta));
+        const float gm = (float)(hd * B.cfg.kappa / (hd * B.cfg.kappa + B.cfg.beta));
+        B.contact_setup(eps, gm, (float)(1.0 / hd));
+        std::vector<float> pf(p, p + 3 * (size_t)B.n_all);
+        B.p.upload(pf.data(), pf.size(), B.stream);
+        // p already holds (x - x0 + Uc)/h here, and r_c = 0: run the sweeps only
+        for (int g = 0; g < B.cfg.gs_max_iter && B.nc > 0; g++) B.contact_sweep_once();
+        B.p.download(pf.data(), pf.size(), B.stream);
+        std::vector<float4> r(B.nc);
+        if (B.nc) B.rc.download(r.data(), B.nc, B.stream);
+        PA_HIP(hipStreamSynchronize(B.stream));
+        for (size_t i = 0; i < pf.size(); i++) p[i] = pf[i];
+        if (rco)
+            for (int c = 0; c < B.nc; c++) {
+                rco[3 * c] = r[c].x;
+                rco[3 * c + 1] = r[c].y;
+                rco[3 * c + 2] = r[c].z;
+            }
+    });
+}
+
+pa_status pa_backend_global_solve(pa_backend* b, const double* rhs, const double* x_init, double* xo, int32_t* iters) {
+    BK({
+        pa::Backend& B = *b->b;
+        PA_CHECK(B.initialized, PA_ERR_INVALID, "init first");
+        PA_CHECK(rhs && x_init && xo, PA_ERR_INVALID, "null argument");
+        const size_t nn = 3 * (size_t)B.n;
+        std::vector<float> xf(x_init, x_init + nn);
+        B.b.upload(rhs, nn, B.stream);
+        B.x.upload(xf.data(), nn, B.stream);
+        int it = B.global_solve(B.cfg.global_max_iter, B.cfg.global_tol);
+        B.x.download(xf.data(), nn, B.stream);
+        PA_HIP(hipStreamSynchronize(B.stream));
+        for (size_t i = 0; i < nn; i++) xo[i] = xf[i];
+        if (iters) *iters = it;
+    });
+}
+
+pa_status pa_backend_get_sizes(pa_backend* b, int64_t* o) {
+    BK({
+        PA_CHECK(o, PA_ERR_INVALID, "null argument");
+        pa::Backend& B = *b->b;
+        o[0] = B.n_all;
+        o[1] = B.n;
+        o[2] = B.m;
+        o[3] = B.nnz_a_off;
+        o[4] = B.nnz_dm;
+        o[5] = (int64_t)B.dt_row.n;
+        o[6] = B.nc;
+    });
+}
+
+pa_status pa_backend_phase_times(pa_backend* b, double* ms6) {
+    BK(PA_CHECK(ms6, PA_ERR_INVALID, "null argument"); for (int k = 0; k < 6; k++) ms6[k] = b->b->phase_ms[k]);
+}
+
+pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, double* avg_ms, int64_t* launches) {
+    BK({
+        PA_CHECK(which == 0, PA_ERR_INVALID, "unknown kernel id");
+        pa::Backend& B = *b->b;
+        if (avg_ms) *avg_ms = B.spmv_launches ? B.spmv_ms_sum / (double)B.spmv_launches : 0.0;
+        if (launches) *launches = B.spmv_launches;
+        if (reset) {
+            B.spmv_ms_sum = 0;
+            B.spmv_launches = 0;
+        }
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,108 @@
+// Shared host/device helpers for the gfx950 backend: error propagation,
+// RAII device buffers, small vector math.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "pa_admm.h"
+
+namespace pa {
+
+// Internal errors carry a pa_status; the C ABI converts them to return codes.
+struct Error : std::runtime_error {
+    pa_status code;
+    Error(pa_status c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define PA_HIP(call)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            throw ::pa::Error(PA_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_) + " (" __FILE__ ":" + \
+                                              std::to_string(__LINE__) + ")");                                \
+    } while (0)
+
+#define PA_CHECK(cond, code, msg)                                                                             \
+    do {                                                                                                      \
+        if (!(cond)) throw ::pa::Error((code), (msg));                                                        \
+    } while (0)
+
+#define PA_LAUNCH_CHECK() PA_HIP(hipGetLastError())
+
+// Device buffer owning `n` elements of T.
+template <class T> struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { alloc(count); }
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) PA_HIP(hipMalloc(&p, sizeof(T) * count));
+    }
+    // Grow-only reallocation (contents not preserved).
+    void reserve(size_t count) {
+        if (count > n) alloc(count);
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        PA_CHECK(count <= n, PA_ERR_INVALID, "upload larger than buffer");
+        if (count) PA_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) {
+        reserve(h.size());
+        upload(h.data(), h.size(), s);
+    }
+    void download(T* h, size_t count, hipStream_t s) const {
+        PA_CHECK(count <= n, PA_ERR_INVALID, "download larger than buffer");
+        if (count) PA_HIP(hipMemcpyAsync(h, p, sizeof(T) * count, hipMemcpyDeviceToHost, s));
+    }
+    void zero(hipStream_t s) {
+        if (n) PA_HIP(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+} // namespace pa
+
+// ---------------------------------------------------------------------------
+// float3 math (device and host)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline float3 f3(float x, float y, float z) { return make_float3(x, y, z); }
+__host__ __device__ inline float3 operator+(float3 a, float3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline float3 operator-(float3 a, float3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline float3 operator*(float s, float3 a) { return f3(s * a.x, s * a.y, s * a.z); }
+__host__ __device__ inline float dot3(float3 a, float3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ inline float3 ld3(const float* p, int64_t i) { return f3(p[3 * i], p[3 * i + 1], p[3 * i + 2]); }
+__device__ inline void st3(float* p, int64_t i, float3 v) {
+    p[3 * i] = v.x;
+    p[3 * i + 1] = v.y;
+    p[3 * i + 2] = v.z;
+}

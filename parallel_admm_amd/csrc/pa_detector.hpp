@@ -1,0 +1,63 @@
+// LBVH proximity detector (replaces BVH_GPU + lbvh_f/lbvh_e,
+// src/contact/collision_detection_cuda.*, src/contact/lbvh/lbvh.cu).
+#pragma once
+
+#include "pa_common.hpp"
+#include "pa_prims.hpp"
+
+namespace pa {
+
+// One Karras radix tree over N primitives: internal nodes [0, N-1),
+// leaves [N-1, 2N-1); node boxes as (lo, hi) float4 pairs.
+struct Tree {
+    int n = 0;
+    DBuf<float4> prim_box;   // 2 per primitive (lo, hi)
+    DBuf<float4> node_box;   // 2 per node
+    DBuf<uint32_t> keys, ids; // sorted Morton codes and primitive ids
+    DBuf<int32_t> child;     // 2 per internal node
+    DBuf<int32_t> parent;    // per node
+    DBuf<uint32_t> flags;    // per internal node (bottom-up visit counters)
+    DBuf<float4> scene;      // 2: scene box
+    DBuf<float4> partial;    // 2 per reduction block
+};
+
+// Contact list in canonical order (PT by (vertex, face), then EE by (edge i, edge j)).
+struct ContactBuffers {
+    int cap = 0;
+    DBuf<int32_t> type, prim0, prim1;
+    DBuf<int4> v;        // vertex ids
+    DBuf<float4> bary;   // 4 signed weights
+    DBuf<float4> nrm;    // normal xyz, h_cN in w
+    DBuf<double> dist;
+};
+
+struct Detector {
+    hipStream_t stream = nullptr;     // stream work is issued on (the backend's once attached)
+    hipStream_t own_stream = nullptr; // created here when none was given; destroyed here
+    int n_all = 0, n_dyn = 0, nv = 0, nf = 0, ne = 0;
+    double dhat = 0;
+    DBuf<int32_t> surf_verts, tris, edges;
+    DBuf<float> x_own; // positions uploaded through detect(host x)
+    Tree ft, et;
+    ContactBuffers c;
+    DBuf<uint32_t> qcount, qoffset; // nv + ne query counts / offsets
+    DBuf<uint32_t> total;           // [0] = contacts found (may exceed cap), [1] = stack overflow flag
+    PrimScratch scratch;
+    int n_contacts = 0, peak = 0;
+    int64_t found_total = 0; // contacts found by the last detection (may exceed cap)
+    bool stack_overflow = false;
+
+    Detector(int n_all, int n_dyn, int nv, const int32_t* sv, int nf, const int32_t* t, int ne, const int32_t* e, int cap,
+             double dhat, hipStream_t s);
+    ~Detector();
+    // Runs detection on device positions x (3 n_all floats, on this->stream).
+    // Returns the number of contacts stored; sets found_total and throws
+    // PA_ERR_OVERFLOW after storing `cap` contacts if more were found.
+    int detect(const float* x);
+    void set_stream(hipStream_t s) { stream = s; }
+
+  private:
+    void build_tree(Tree& T, const float* x, bool faces);
+};
+
+} // namespace pa

@@ -1,0 +1,260 @@
+// FP64 proximity predicates for the detector, evaluated with IEEE operations
+// only (no contraction; +,-,*,/,sqrt are correctly rounded on gfx950 and on
+// the host), so the accept/reject decision and the contact fields are
+// reproducible bit for bit.
+//
+// Point-triangle: D. Eberly, "Distance Between Point and Triangle in 3D",
+// Geometric Tools (region decomposition of the (s,t) domain), as used by the
+// reference's DistPT (src/mutils/distPT.h:59-267).
+// Segment-segment: D. Eberly, "Distance Between Two Line Segments", Geometric
+// Tools (robust (s,t) region walk with a separate parallel case), as used by
+// the reference's DistEE (src/mutils/distEE.h:62-300).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#pragma clang fp contract(off)
+
+namespace pa {
+
+struct dv3 {
+    double x, y, z;
+};
+__host__ __device__ inline dv3 dmk(double x, double y, double z) { return dv3{x, y, z}; }
+__host__ __device__ inline dv3 dsub(dv3 a, dv3 b) { return dv3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline dv3 dadd(dv3 a, dv3 b) { return dv3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline dv3 dscale(double s, dv3 a) { return dv3{s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline double ddot(dv3 a, dv3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__host__ __device__ inline dv3 dcross(dv3 a, dv3 b) {
+    return dv3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline dv3 dnormalized(dv3 a) {
+    double n2 = ddot(a, a);
+    if (n2 > 0.0) {
+        double n = sqrt(n2);
+        return dv3{a.x / n, a.y / n, a.z / n};
+    }
+    return a;
+}
+
+struct PTOut {
+    double dist;
+    double b0, b1, b2;   // closest = b0 t0 + b1 t1 + b2 t2
+    dv3 c1;              // closest point on the triangle
+};
+
+__host__ __device__ inline PTOut point_triangle(dv3 p, dv3 t0, dv3 t1, dv3 t2) {
+    const double zero = 0.0, one = 1.0, two = 2.0;
+    dv3 diff = dsub(t0, p), e0 = dsub(t1, t0), e1 = dsub(t2, t0);
+    double a00 = ddot(e0, e0), a01 = ddot(e0, e1), a11 = ddot(e1, e1);
+    double b0 = ddot(diff, e0), b1 = ddot(diff, e1);
+    double det = a00 * a11 - a01 * a01;
+    det = (det < zero) ? zero : det; // std::max(det, 0) semantics
+    double s = a01 * b1 - a11 * b0;
+    double t = a01 * b0 - a00 * b1;
+    if (s + t <= det) {
+        if (s < zero) {
+            if (t < zero) { // region 4
+                if (b0 < zero) {
+                    t = zero;
+                    s = (-b0 >= a00) ? one : -b0 / a00;
+                } else {
+                    s = zero;
+                    t = (b1 >= zero) ? zero : ((-b1 >= a11) ? one : -b1 / a11);
+                }
+            } else { // region 3
+                s = zero;
+                t = (b1 >= zero) ? zero : ((-b1 >= a11) ? one : -b1 / a11);
+            }
+        } else if (t < zero) { // region 5
+            t = zero;
+            s = (b0 >= zero) ? zero : ((-b0 >= a00) ? one : -b0 / a00);
+        } else { // region 0
+            s = s / det;
+            t = t / det;
+        }
+    } else if (s < zero) { // region 2
+        double tmp0 = a01 + b0, tmp1 = a11 + b1;
+        if (tmp1 > tmp0) {
+            double numer = tmp1 - tmp0, denom = a00 - two * a01 + a11;
+            if (numer >= denom) { s = one; t = zero; }
+            else { s = numer / denom; t = one - s; }
+        } else {
+            s = zero;
+            t = (tmp1 <= zero) ? one : ((b1 >= zero) ? zero : -b1 / a11);
+        }
+    } else if (t < zero) { // region 6
+        double tmp0 = a01 + b1, tmp1 = a00 + b0;
+        if (tmp1 > tmp0) {
+            double numer = tmp1 - tmp0, denom = a00 - two * a01 + a11;
+            if (numer >= denom) { t = one; s = zero; }
+            else { t = numer / denom; s = one - t; }
+        } else {
+            t = zero;
+            s = (tmp1 <= zero) ? one : ((b0 >= zero) ? zero : -b0 / a00);
+        }
+    } else { // region 1
+        double numer = a11 + b1 - a01 - b0;
+        if (numer <= zero) { s = zero; t = one; }
+        else {
+            double denom = a00 - two * a01 + a11;
+            if (numer >= denom) { s = one; t = zero; }
+            else { s = numer / denom; t = one - s; }
+        }
+    }
+    PTOut r;
+    r.c1 = dadd(dadd(t0, dscale(s, e0)), dscale(t, e1));
+    dv3 d = dsub(p, r.c1);
+    r.dist = sqrt(ddot(d, d));
+    r.b0 = (one - s) - t;
+    r.b1 = s;
+    r.b2 = t;
+    return r;
+}
+
+struct EEOut {
+    double dist;
+    double s, t;
+    dv3 c0, c1;
+};
+
+__host__ __device__ inline EEOut segment_segment(dv3 P0, dv3 P1, dv3 Q0, dv3 Q1) {
+    const double zero = 0.0, one = 1.0;
+    dv3 P = dsub(P1, P0), Q = dsub(Q1, Q0), PQ = dsub(P0, Q0);
+    double a = ddot(P, P), b = ddot(P, Q), c = ddot(Q, Q), d = ddot(P, PQ), e = ddot(Q, PQ);
+    double det = a * c - b * b;
+    double s = zero, t = zero;
+    if (det > zero) {
+        double bte = b * e, ctd = c * d;
+        if (bte <= ctd) {
+            s = zero;
+            if (e <= zero) { // region 6
+                t = zero;
+                double nd = -d;
+                if (nd >= a) s = one;
+                else if (nd > zero) s = nd / a;
+            } else if (e < c) { // region 5
+                t = e / c;
+            } else { // region 4
+                t = one;
+                double bmd = b - d;
+                if (bmd >= a) s = one;
+                else if (bmd > zero) s = bmd / a;
+            }
+        } else {
+            s = bte - ctd;
+            if (s >= det) {
+                s = one;
+                double bpe = b + e;
+                if (bpe <= zero) { // region 8
+                    t = zero;
+                    double nd = -d;
+                    if (nd <= zero) s = zero;
+                    else if (nd < a) s = nd / a;
+                } else if (bpe < c) { // region 1
+                    t = bpe / c;
+                } else { // region 2
+                    t = one;
+                    double bmd = b - d;
+                    if (bmd <= zero) s = zero;
+                    else if (bmd < a) s = bmd / a;
+                }
+            } else {
+                double ate = a * e, btd = b * d;
+                if (ate <= btd) { // region 7
+                    t = zero;
+                    double nd = -d;
+                    s = (nd <= zero) ? zero : ((nd >= a) ? one : nd / a);
+                } else {
+                    t = ate - btd;
+                    if (t >= det) { // region 3
+                        t = one;
+                        double bmd = b - d;
+                        s = (bmd <= zero) ? zero : ((bmd >= a) ? one : bmd / a);
+                    } else { // region 0
+                        s = s / det;
+                        t = t / det;
+                    }
+                }
+            }
+        }
+    } else { // parallel
+        if (e <= zero) {
+            t = zero;
+            double nd = -d;
+            s = (nd <= zero) ? zero : ((nd >= a) ? one : nd / a);
+        } else if (e >= c) {
+            t = one;
+            double bmd = b - d;
+            s = (bmd <= zero) ? zero : ((bmd >= a) ? one : bmd / a);
+        } else {
+            s = zero;
+            t = e / c;
+        }
+    }
+    EEOut r;
+    r.s = s;
+    r.t = t;
+    r.c0 = dadd(P0, dscale(s, P));
+    r.c1 = dadd(Q0, dscale(t, Q));
+    dv3 dd = dsub(r.c0, r.c1);
+    r.dist = sqrt(ddot(dd, dd));
+    return r;
+}
+
+// Contact record fields (reference ContactInfo, src/contact/contact_info.h:116-177):
+// PT bary (1, -b0, -b1, -b2), normal = unit((t1 - t0) x (t2 - t0));
+// EE bary (1 - s, s, t - 1, -t), normal = unit((P1 - P0) x (Q1 - Q0));
+// the normal is flipped so that n . (c0 - c1) >= 0, and h_cN = n . (c0 - c1) - dhat.
+struct ContactFields {
+    double bary[4];
+    dv3 n;
+    double h_cN;
+};
+
+__host__ __device__ inline ContactFields pt_fields(dv3 p, dv3 t0, dv3 t1, dv3 t2, const PTOut& r, double dhat) {
+    ContactFields f;
+    dv3 n = dnormalized(dcross(dsub(t1, t0), dsub(t2, t0)));
+    f.bary[0] = 1.0;
+    f.bary[1] = -r.b0;
+    f.bary[2] = -r.b1;
+    f.bary[3] = -r.b2;
+    dv3 h = dsub(p, r.c1);
+    double hn = ddot(n, h);
+    if (hn < 0.0) {
+        n = dscale(-1.0, n);
+        hn = -hn;
+    }
+    f.n = n;
+    f.h_cN = hn - dhat;
+    return f;
+}
+
+__host__ __device__ inline ContactFields ee_fields(dv3 P0, dv3 P1, dv3 Q0, dv3 Q1, const EEOut& r, double dhat) {
+    ContactFields f;
+    dv3 n = dnormalized(dcross(dsub(P1, P0), dsub(Q1, Q0)));
+    f.bary[0] = 1.0 - r.s;
+    f.bary[1] = r.s;
+    f.bary[2] = r.t - 1.0;
+    f.bary[3] = -r.t;
+    dv3 h = dsub(r.c0, r.c1);
+    double hn = ddot(n, h);
+    if (hn < 0.0) {
+        n = dscale(-1.0, n);
+        hn = -hn;
+    }
+    f.n = n;
+    f.h_cN = hn - dhat;
+    return f;
+}
+
+// ProximalQuery acceptance (narrow_phase.cpp:387, 455-462).
+__host__ __device__ inline bool pt_accept(const PTOut& r, double dhat) { return r.dist <= dhat; }
+__host__ __device__ inline bool ee_accept(const EEOut& r, double dhat) {
+    if (!(r.dist < dhat)) return false;
+    bool s_end = r.s == 0.0 || r.s == 1.0;
+    bool t_end = r.t == 0.0 || r.t == 1.0;
+    return !(s_end && t_end);
+}
+
+} // namespace pa

@@ -1,0 +1,478 @@
+// LBVH proximity detection on gfx950.
+//
+// Per detection: primitive boxes -> scene box -> 30-bit Morton codes ->
+// stable LDS radix sort -> Karras radix tree (T. Karras, "Maximizing
+// Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
+// -> bottom-up boxes -> two traversal passes (count, then write into
+// scanned per-query segments) with the FP64 predicate.  Each query sorts its
+// own segment by partner id, so the list comes out in canonical order with
+// no global sort and no atomics on the contact list.
+//
+// Reference: lbvh_f/lbvh_e::Construct (src/contact/lbvh/lbvh.cu:1468-1540),
+// _dcd_vf/_dcd_ee (:1694-1813); acceptance follows ProximalQuery
+// (src/contact/narrow_phase.cpp:192-245, 313-469) rather than the reference
+// GPU's interior-only test, so the GPU set equals the CPU set.
+#include "pa_detector.hpp"
+#include "pa_geom.hpp"
+
+#include <algorithm>
+
+#pragma clang fp contract(off)
+
+namespace pa {
+namespace {
+
+constexpr int BLK = 256;
+constexpr int QBLK = 128;
+constexpr int STACK = 64;
+
+__device__ inline float4 fmin4(float4 a, float4 b) { return make_float4(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z), 0.f); }
+__device__ inline float4 fmax4(float4 a, float4 b) { return make_float4(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z), 0.f); }
+
+// Boxes of triangles (k = 3) or edges (k = 2).
+__global__ void k_prim_boxes(const float* __restrict__ x, const int32_t* __restrict__ prims, int k, int n,
+                             float4* __restrict__ box) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int q = 0; q < k; q++) {
+        int v = prims[k * i + q];
+        float4 p = make_float4(x[3 * v], x[3 * v + 1], x[3 * v + 2], 0.f);
+        lo = fmin4(lo, p);
+        hi = fmax4(hi, p);
+    }
+    box[2 * i] = lo;
+    box[2 * i + 1] = hi;
+}
+
+__device__ inline float4 wave_min4(float4 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x = fminf(v.x, __shfl_xor(v.x, o, 64));
+        v.y = fminf(v.y, __shfl_xor(v.y, o, 64));
+        v.z = fminf(v.z, __shfl_xor(v.z, o, 64));
+    }
+    return v;
+}
+__device__ inline float4 wave_max4(float4 v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        v.x = fmaxf(v.x, __shfl_xor(v.x, o, 64));
+        v.y = fmaxf(v.y, __shfl_xor(v.y, o, 64));
+        v.z = fmaxf(v.z, __shfl_xor(v.z, o, 64));
+    }
+    return v;
+}
+
+// Union of n boxes (2 float4 each) per block -> out[2 * blockIdx].
+__global__ void __launch_bounds__(BLK) k_box_union(const float4* __restrict__ box, int n, float4* __restrict__ out) {
+    __shared__ float4 slo[BLK / 64], shi[BLK / 64];
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        lo = fmin4(lo, box[2 * i]);
+        hi = fmax4(hi, box[2 * i + 1]);
+    }
+    lo = wave_min4(lo);
+    hi = wave_max4(hi);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        slo[wave] = lo;
+        shi[wave] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < BLK / 64; w++) {
+            lo = fmin4(lo, slo[w]);
+            hi = fmax4(hi, shi[w]);
+        }
+        out[2 * blockIdx.x] = lo;
+        out[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+__device__ inline uint32_t spread10(uint32_t v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__device__ inline uint32_t quant10(float u) {
+    float q = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    return (uint32_t)q;
+}
+
+__global__ void k_morton(const float4* __restrict__ box, int n, const float4* __restrict__ scene, uint32_t* __restrict__ keys,
+                         uint32_t* __restrict__ ids) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 slo = scene[0], shi = scene[1];
+    float4 lo = box[2 * i], hi = box[2 * i + 1];
+    float ex = shi.x - slo.x, ey = shi.y - slo.y, ez = shi.z - slo.z;
+    float cx = 0.5f * (lo.x + hi.x) - slo.x, cy = 0.5f * (lo.y + hi.y) - slo.y, cz = 0.5f * (lo.z + hi.z) - slo.z;
+    uint32_t qx = quant10(ex > 0.f ? cx / ex : 0.f);
+    uint32_t qy = quant10(ey > 0.f ? cy / ey : 0.f);
+    uint32_t qz = quant10(ez > 0.f ? cz / ez : 0.f);
+    keys[i] = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
+    ids[i] = (uint32_t)i;
+}
+
+__device__ inline int kdelta(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = ((uint64_t)keys[i] << 32) | ids[i];
+    uint64_t b = ((uint64_t)keys[j] << 32) | ids[j];
+    return __clzll(a ^ b);
+}
+
+// Karras construction of internal node i over sorted unique 64-bit keys.
+__global__ void k_karras(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids, int n, int32_t* __restrict__ child,
+                         int32_t* __restrict__ parent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (kdelta(keys, ids, n, i, i + 1) - kdelta(keys, ids, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(keys, ids, n, i, i - d);
+    int lmax = 2;
+    while (kdelta(keys, ids, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (kdelta(keys, ids, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(keys, ids, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (kdelta(keys, ids, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int g = i + s * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int left = (lo == g) ? (n - 1) + g : g;
+    const int right = (hi == g + 1) ? (n - 1) + g + 1 : g + 1;
+    child[2 * i] = left;
+    child[2 * i + 1] = right;
+    parent[left] = i;
+    parent[right] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+// Leaf boxes, then climb: the second thread to reach a node writes its box.
+__global__ void k_bottom_up(const float4* __restrict__ prim_box, const uint32_t* __restrict__ ids, int n,
+                            const int32_t* __restrict__ child, const int32_t* __restrict__ parent, uint32_t* __restrict__ flags,
+                            float4* __restrict__ node_box) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    int node = (n - 1) + k;
+    uint32_t pid = ids[k];
+    float4 lo = prim_box[2 * pid], hi = prim_box[2 * pid + 1];
+    node_box[2 * node] = lo;
+    node_box[2 * node + 1] = hi;
+    if (n == 1) return;
+    int p = parent[node];
+    while (p >= 0) {
+        __threadfence();
+        uint32_t old = atomicAdd(&flags[p], 1u);
+        __threadfence();
+        if (old == 0) return;
+        int a = child[2 * p], b = child[2 * p + 1];
+        float4 alo = node_box[2 * a], ahi = node_box[2 * a + 1];
+        float4 blo = node_box[2 * b], bhi = node_box[2 * b + 1];
+        lo = fmin4(alo, blo);
+        hi = fmax4(ahi, bhi);
+        node_box[2 * p] = lo;
+        node_box[2 * p + 1] = hi;
+        p = parent[p];
+    }
+}
+
+__device__ inline bool box_far(float4 nlo, float4 nhi, float4 qlo, float4 qhi, float gap) {
+    return (nlo.x - qhi.x > gap) || (qlo.x - nhi.x > gap) || (nlo.y - qhi.y > gap) || (qlo.y - nhi.y > gap) ||
+           (nlo.z - qhi.z > gap) || (qlo.z - nhi.z > gap);
+}
+
+__device__ inline dv3 ldd(const float* x, int v) { return dmk((double)x[3 * v], (double)x[3 * v + 1], (double)x[3 * v + 2]); }
+
+struct QueryOut {
+    int32_t* type;
+    int32_t* prim0;
+    int32_t* prim1;
+    int4* v;
+    float4* bary;
+    float4* nrm;
+    double* dist;
+    int cap;
+};
+
+__device__ inline void write_contact(const QueryOut& o, int idx, int type, int p0, int p1, int4 vv, const ContactFields& f,
+                                     double dist) {
+    o.type[idx] = type;
+    o.prim0[idx] = p0;
+    o.prim1[idx] = p1;
+    o.v[idx] = vv;
+    o.bary[idx] = make_float4((float)f.bary[0], (float)f.bary[1], (float)f.bary[2], (float)f.bary[3]);
+    o.nrm[idx] = make_float4((float)f.n.x, (float)f.n.y, (float)f.n.z, (float)f.h_cN);
+    o.dist[idx] = dist;
+}
+
+// Insertion sort of one query's segment by partner id (prim1).
+__device__ inline void sort_segment(const QueryOut& o, int begin, int end) {
+    for (int a = begin + 1; a < end; a++) {
+        int32_t key = o.prim1[a];
+        int32_t t0 = o.type[a], p0 = o.prim0[a];
+        int4 vv = o.v[a];
+        float4 bb = o.bary[a], nn = o.nrm[a];
+        double dd = o.dist[a];
+        int b = a - 1;
+        while (b >= begin && o.prim1[b] > key) {
+            o.type[b + 1] = o.type[b];
+            o.prim0[b + 1] = o.prim0[b];
+            o.prim1[b + 1] = o.prim1[b];
+            o.v[b + 1] = o.v[b];
+            o.bary[b + 1] = o.bary[b];
+            o.nrm[b + 1] = o.nrm[b];
+            o.dist[b + 1] = o.dist[b];
+            b--;
+        }
+        o.type[b + 1] = t0;
+        o.prim0[b + 1] = p0;
+        o.prim1[b + 1] = key;
+        o.v[b + 1] = vv;
+        o.bary[b + 1] = bb;
+        o.nrm[b + 1] = nn;
+        o.dist[b + 1] = dd;
+    }
+}
+
+// Point-triangle queries: one lane per surface vertex, LDS stack per lane.
+// write == 0: count accepted pairs into qcount; write == 1: store them at
+// qoffset and sort the segment.
+__global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __restrict__ x, const int32_t* __restrict__ surf_verts,
+                                                   int nv, const int32_t* __restrict__ tris, int n_dyn, int nf,
+                                                   const float4* __restrict__ node_box, const int32_t* __restrict__ child,
+                                                   const uint32_t* __restrict__ ids, double dhat, float gap,
+                                                   uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoffset,
+                                                   QueryOut o, uint32_t* __restrict__ flags) {
+    __shared__ int32_t stack[STACK][QBLK];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nv || nf == 0) {
+        if (q < nv && !write) qcount[q] = 0;
+        return;
+    }
+    const int v = surf_verts[q];
+    const float4 pf = make_float4(x[3 * v], x[3 * v + 1], x[3 * v + 2], 0.f);
+    const dv3 p = dmk((double)pf.x, (double)pf.y, (double)pf.z);
+    const bool vstatic = v >= n_dyn;
+    const int leaf0 = nf - 1;
+    int count = 0;
+    const int base = write ? (int)qoffset[q] : 0;
+    int sp = 0;
+    int node = 0; // root (a leaf when nf == 1)
+    while (true) {
+        if (node >= leaf0) {
+            const int f = (int)ids[node - leaf0];
+            const int a = tris[3 * f], b = tris[3 * f + 1], c = tris[3 * f + 2];
+            if (v != a && v != b && v != c && !(vstatic && a >= n_dyn)) {
+                const dv3 t0 = ldd(x, a), t1 = ldd(x, b), t2 = ldd(x, c);
+                const PTOut r = point_triangle(p, t0, t1, t2);
+                if (pt_accept(r, dhat)) {
+                    if (write) {
+                        const int idx = base + count;
+                        if (idx < o.cap) write_contact(o, idx, 0, v, f, make_int4(v, a, b, c), pt_fields(p, t0, t1, t2, r, dhat), r.dist);
+                    }
+                    count++;
+                }
+            }
+        } else {
+            const int l = child[2 * node], r = child[2 * node + 1];
+            const bool hl = !box_far(node_box[2 * l], node_box[2 * l + 1], pf, pf, gap);
+            const bool hr = !box_far(node_box[2 * r], node_box[2 * r + 1], pf, pf, gap);
+            if (hl && hr) {
+                if (sp < STACK) stack[sp++][threadIdx.x] = r;
+                else atomicOr(&flags[1], 1u);
+                node = l;
+                continue;
+            }
+            if (hl) { node = l; continue; }
+            if (hr) { node = r; continue; }
+        }
+        if (sp == 0) break;
+        node = stack[--sp][threadIdx.x];
+    }
+    if (!write) {
+        qcount[q] = (uint32_t)count;
+    } else {
+        const int end = min(base + count, o.cap);
+        sort_segment(o, base, end);
+    }
+}
+
+// Edge-edge queries: one lane per surface edge i, partners j > i.
+__global__ void __launch_bounds__(QBLK) k_query_ee(int write, const float* __restrict__ x, const int32_t* __restrict__ edges, int ne,
+                                                   int n_dyn, const float4* __restrict__ node_box,
+                                                   const int32_t* __restrict__ child, const uint32_t* __restrict__ ids,
+                                                   double dhat, float gap, uint32_t* __restrict__ qcount,
+                                                   const uint32_t* __restrict__ qoffset, QueryOut o, uint32_t* __restrict__ flags) {
+    __shared__ int32_t stack[STACK][QBLK];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ne) return;
+    const int a0 = edges[2 * i], a1 = edges[2 * i + 1];
+    const float4 pa0 = make_float4(x[3 * a0], x[3 * a0 + 1], x[3 * a0 + 2], 0.f);
+    const float4 pa1 = make_float4(x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], 0.f);
+    const float4 qlo = fmin4(pa0, pa1), qhi = fmax4(pa0, pa1);
+    const dv3 P0 = dmk(pa0.x, pa0.y, pa0.z), P1 = dmk(pa1.x, pa1.y, pa1.z);
+    const bool astatic = a0 >= n_dyn;
+    const int leaf0 = ne - 1;
+    int count = 0;
+    const int base = write ? (int)qoffset[i] : 0;
+    int sp = 0;
+    int node = 0;
+    while (true) {
+        if (node >= leaf0) {
+            const int j = (int)ids[node - leaf0];
+            if (j > i) {
+                const int b0 = edges[2 * j], b1 = edges[2 * j + 1];
+                if (a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && !(astatic && b0 >= n_dyn)) {
+                    const dv3 Q0 = ldd(x, b0), Q1 = ldd(x, b1);
+                    const EEOut r = segment_segment(P0, P1, Q0, Q1);
+                    if (ee_accept(r, dhat)) {
+                        if (write) {
+                            const int idx = base + count;
+                            if (idx < o.cap)
+                                write_contact(o, idx, 1, i, j, make_int4(a0, a1, b0, b1), ee_fields(P0, P1, Q0, Q1, r, dhat), r.dist);
+                        }
+                        count++;
+                    }
+                }
+            }
+        } else {
+            const int l = child[2 * node], r = child[2 * node + 1];
+            const bool hl = !box_far(node_box[2 * l], node_box[2 * l + 1], qlo, qhi, gap);
+            const bool hr = !box_far(node_box[2 * r], node_box[2 * r + 1], qlo, qhi, gap);
+            if (hl && hr) {
+                if (sp < STACK) stack[sp++][threadIdx.x] = r;
+                else atomicOr(&flags[1], 1u);
+                node = l;
+                continue;
+            }
+            if (hl) { node = l; continue; }
+            if (hr) { node = r; continue; }
+        }
+        if (sp == 0) break;
+        node = stack[--sp][threadIdx.x];
+    }
+    if (!write) {
+        qcount[i] = (uint32_t)count;
+    } else {
+        const int end = min(base + count, o.cap);
+        sort_segment(o, base, end);
+    }
+}
+
+} // namespace
+
+Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, const int32_t* t, int ne_, const int32_t* e,
+                   int cap, double dhat_, hipStream_t s)
+    : stream(s), n_all(n_all_), n_dyn(n_dyn_), nv(nv_), nf(nf_), ne(ne_), dhat(dhat_) {
+    PA_CHECK(cap > 0, PA_ERR_INVALID, "detector capacity must be positive");
+    PA_CHECK(dhat > 0, PA_ERR_INVALID, "thickness must be positive");
+    if (!stream) {
+        PA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+        stream = own_stream;
+    }
+    for (int i = 0; i < nv; i++) PA_CHECK(sv[i] >= 0 && sv[i] < n_all, PA_ERR_INVALID, "surface vertex out of range");
+    for (int i = 0; i < 3 * nf; i++) PA_CHECK(t[i] >= 0 && t[i] < n_all, PA_ERR_INVALID, "surface triangle index out of range");
+    for (int i = 0; i < 2 * ne; i++) PA_CHECK(e[i] >= 0 && e[i] < n_all, PA_ERR_INVALID, "surface edge index out of range");
+    surf_verts.alloc(nv);
+    surf_verts.upload(sv, nv, stream);
+    tris.alloc(3 * (size_t)nf);
+    tris.upload(t, 3 * (size_t)nf, stream);
+    edges.alloc(2 * (size_t)ne);
+    edges.upload(e, 2 * (size_t)ne, stream);
+    auto init_tree = [&](Tree& T, int n) {
+        T.n = n;
+        if (n == 0) return;
+        T.prim_box.alloc(2 * (size_t)n);
+        T.node_box.alloc(2 * (size_t)(2 * n - 1));
+        T.keys.alloc(n);
+        T.ids.alloc(n);
+        T.child.alloc(2 * (size_t)std::max(n - 1, 1));
+        T.parent.alloc(2 * (size_t)n - 1);
+        T.flags.alloc(std::max(n - 1, 1));
+        T.scene.alloc(2);
+        T.partial.alloc(2 * 256);
+    };
+    init_tree(ft, nf);
+    init_tree(et, ne);
+    c.cap = cap;
+    c.type.alloc(cap);
+    c.prim0.alloc(cap);
+    c.prim1.alloc(cap);
+    c.v.alloc(cap);
+    c.bary.alloc(cap);
+    c.nrm.alloc(cap);
+    c.dist.alloc(cap);
+    qcount.alloc((size_t)nv + ne + 1);
+    qoffset.alloc((size_t)nv + ne + 1);
+    total.alloc(2);
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+Detector::~Detector() {
+    if (own_stream) {
+        (void)hipStreamSynchronize(own_stream);
+        (void)hipStreamDestroy(own_stream);
+    }
+}
+
+void Detector::build_tree(Tree& T, const float* x, bool faces) {
+    const int n = T.n;
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_prim_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, x, faces ? tris.p : edges.p, faces ? 3 : 2, n,
+                       T.prim_box.p);
+    PA_LAUNCH_CHECK();
+    const int nb = std::min(256, grid_for(n, BLK));
+    hipLaunchKernelGGL(k_box_union, dim3(nb), dim3(BLK), 0, stream, T.prim_box.p, n, T.partial.p);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_box_union, dim3(1), dim3(BLK), 0, stream, T.partial.p, nb, T.scene.p);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_morton, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, n, T.scene.p, T.keys.p, T.ids.p);
+    PA_LAUNCH_CHECK();
+    radix_sort_pairs(T.keys.p, T.ids.p, n, 30, scratch, stream);
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, T.keys.p, T.ids.p, n, T.child.p, T.parent.p);
+        PA_LAUNCH_CHECK();
+        fill<uint32_t>(T.flags.p, 0u, n - 1, stream);
+    }
+    hipLaunchKernelGGL(k_bottom_up, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, T.ids.p, n, T.child.p, T.parent.p,
+                       T.flags.p, T.node_box.p);
+    PA_LAUNCH_CHECK();
+}
+
+int Detector::detect(const float* x) {
+    build_tree(ft, x, true);
+    build_tree(et, x, false);
+    fill<uint32_t>(total.p, 0u, 2, stream);
+    const float gap = (float)(dhat * 1.001);
+    QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
+    for (int write = 0; write < 2; write++) {
+        if (nv > 0)
+            hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, write, x, surf_verts.p, nv, tris.p, n_dyn, nf,
+                               ft.node_box.p, ft.child.p, ft.ids.p, dhat, gap, qcount.p, qoffset.p, o, total.p);
+        PA_LAUNCH_CHECK();
+        if (ne > 0)
+            hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, write, x, edges.p, ne, n_dyn, et.node_box.p,
+                               et.child.p, et.ids.p, dhat, gap, qcount.p + nv, qoffset.p + nv, o, total.p);
+        PA_LAUNCH_CHECK();
+        if (write == 0) scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);
+    }
+    uint32_t h[2];
+    PA_HIP(hipMemcpyAsync(h, total.p, sizeof(h), hipMemcpyDeviceToHost, stream));
+    PA_HIP(hipStreamSynchronize(stream));
+    found_total = h[0];
+    stack_overflow = h[1] != 0;
+    n_contacts = (int)std::min<int64_t>(found_total, c.cap);
+    peak = std::max(peak, n_contacts);
+    PA_CHECK(!stack_overflow, PA_ERR_INVALID, "LBVH traversal stack overflow");
+    if (found_total > c.cap)
+        throw Error(PA_ERR_OVERFLOW, "contact capacity exceeded: found " + std::to_string(found_total) + " > " + std::to_string(c.cap));
+    return n_contacts;
+}
+
+} // namespace pa

@@ -1,0 +1,1488 @@
+// ADMM time step on gfx950.
+//
+// One time step (reference ADMMImplGPU::step, src/solver/backend_gpu/admm_impl_gpu.cu:256-344,
+// with damping kept in the matrix and the right-hand side as the CPU backend
+// does, src/solver/backend_cpu/admm_impl_cpu.cpp:264-359):
+//   prologue        k_begin_step (+ k_spmv_add for Damp x0)
+//   per iteration   [detection every dcd_interval]
+//                   k_local        z = prox(D x + Ue) per constraint (tri/tet/bend/pin in one launch)
+//                   k_vertex_p     p = (x - x0 + Uc)/h + Gamma r_c      (contact on)
+//                   G x (k_contact_du, k_vertex_gather)                  (friction-cone sweeps)
+//                   k_rhs          b = M x~ + Damp x0 + h^2 D^T We^2 (z - Ue) + h^2 Wc (h p + x0 - Uc)
+//                   PCG            A x = b, Jacobi preconditioner, FP64 dot products
+//                   k_dual_elem / k_dual_vert   Ue += D x - z, Uc += x - x0 - h p (+ residual sums)
+//   epilogue        k_end_step     v = (x - x0)/h
+// D is applied matrix-free from the constraint rest data; D^T We^2 is a
+// per-vertex gather (CSR over vertices) so every sum has a fixed order and the
+// step is deterministic (the reference scatters p with float atomics,
+// src/solver/contact_solver.cu:316-335).
+#include "pa_backend.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace pa {
+namespace {
+
+constexpr int BLK = 256;
+
+// ---------------------------------------------------------------------------
+// Block reduction of K doubles (fixed order -> identical result in every block).
+// ---------------------------------------------------------------------------
+template <int K> __device__ inline void block_sum_write(double (&v)[K], double* __restrict__ part, int stride, int offset) {
+    __shared__ double sh[K][BLK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = v[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sh[k][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            double s = 0;
+            for (int w = 0; w < BLK / 64; w++) s += sh[k][w];
+            part[(size_t)blockIdx.x * stride + offset + k] = s;
+        }
+    }
+}
+
+// Every block sums the same partials in the same order.
+template <int K> __device__ inline void block_reduce_parts(const double* __restrict__ part, int nparts, int stride, int offset,
+                                                           double (&out)[K]) {
+    __shared__ double sh[K][BLK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = 0;
+    for (int b = threadIdx.x; b < nparts; b += BLK)
+#pragma unroll
+        for (int k = 0; k < K; k++) v[k] += part[(size_t)b * stride + offset + k];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = v[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sh[k][wave] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = 0;
+        for (int w = 0; w < BLK / 64; w++) s += sh[k][w];
+        out[k] = s;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Prologue / epilogue (reference do_integration_kernel / do_post_process_kernel,
+// src/solver/jacobi_solver.cu:183-252).
+// ---------------------------------------------------------------------------
+__global__ void k_begin_step(int n_all, int n, float h, float g, const uint8_t* __restrict__ pinned, float* __restrict__ x,
+                             const float* __restrict__ v, float* __restrict__ x0, float* __restrict__ p, double* __restrict__ bconst,
+                             const float* __restrict__ mass) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_all) return;
+    float3 xi = ld3(x, i);
+    st3(x0, i, xi);
+    float3 vi = ld3(v, i);
+    if (i < n) {
+        if (pinned[i]) vi = f3(0.f, 0.f, 0.f);
+        else vi.y -= g * h;
+        float3 xt = xi + h * vi;
+        st3(x, i, xt);
+        const double mi = mass[i];
+        bconst[3 * i] = mi * xt.x;
+        bconst[3 * i + 1] = mi * xt.y;
+        bconst[3 * i + 2] = mi * xt.z;
+    }
+    st3(p, i, vi);
+}
+
+__global__ void k_end_step(int n, float h, const float* __restrict__ x, const float* __restrict__ x0, float* __restrict__ v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 d = ld3(x, i) - ld3(x0, i);
+    st3(v, i, f3(d.x / h, d.y / h, d.z / h));
+}
+
+// y += A x (full CSR, 3 interleaved columns), FP64 accumulation
+__global__ void k_spmv_add(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
+                           const float* __restrict__ xin, double* __restrict__ y) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (int e = ptr[i]; e < ptr[i + 1]; e++) {
+        const double a = val[e];
+        const int c = col[e];
+        s0 += a * xin[3 * c];
+        s1 += a * xin[3 * c + 1];
+        s2 += a * xin[3 * c + 2];
+    }
+    y[3 * i] += s0;
+    y[3 * i + 1] += s1;
+    y[3 * i + 2] += s2;
+}
+
+// ---------------------------------------------------------------------------
+// Elastic constraints: D x per constraint and the prox operators.
+// Reference prox: TriProxFunctor (src/constraint/triangle_constraint_device.cu:111-201),
+// TetProxFunctor (tetrahedral_constraint_device.cu:56-104), BendProxFunctor
+// (bending_constraint_device.cu:31-78), PinProxFunctor (pin_constraint_device.cu:69-93).
+// ---------------------------------------------------------------------------
+struct ElemArgs {
+    int n_tri, n_tet, n_bend, n_pin;
+    int tri_row0, tet_row0, bend_row0, pin_row0;
+    int blk_tri, blk_tet, blk_bend; // block offsets
+    const int4* tri_v;
+    const float4* tri_binv;
+    const float* tri_w;
+    const int4* tet_v;
+    const float* tet_binv;
+    const float* tet_w;
+    const int32_t* bend_ptr;
+    const int32_t* bend_idx;
+    const float* bend_coef;
+    const float* bend_w;
+    const int32_t* pin_v;
+    const float4* pin_pos;
+    const float* pin_w;
+};
+
+// Maps (block, thread) to (type, constraint index).  type: 0 tri, 1 tet, 2 bend, 3 pin, -1 none.
+__device__ inline int elem_of(const ElemArgs& a, int* c) {
+    int b = blockIdx.x;
+    if (b < a.blk_tri) {
+        *c = b * BLK + threadIdx.x;
+        return *c < a.n_tri ? 0 : -1;
+    }
+    if (b < a.blk_tet) {
+        *c = (b - a.blk_tri) * BLK + threadIdx.x;
+        return *c < a.n_tet ? 1 : -1;
+    }
+    if (b < a.blk_bend) {
+        *c = (b - a.blk_tet) * BLK + threadIdx.x;
+        return *c < a.n_bend ? 2 : -1;
+    }
+    *c = (b - a.blk_bend) * BLK + threadIdx.x;
+    return *c < a.n_pin ? 3 : -1;
+}
+
+// D x for constraint c of `type`: rows into dx[], returns number of rows and first row.
+__device__ inline int elem_dx(const ElemArgs& a, int type, int c, const float* __restrict__ x, float3 dx[3], int* row0, float* w) {
+    if (type == 0) {
+        int4 t = a.tri_v[c];
+        float4 B = a.tri_binv[c]; // row-major 2x2
+        float3 xa = ld3(x, t.x), e1 = ld3(x, t.y) - xa, e2 = ld3(x, t.z) - xa;
+        dx[0] = B.x * e1 + B.z * e2;
+        dx[1] = B.y * e1 + B.w * e2;
+        *row0 = a.tri_row0 + 2 * c;
+        *w = a.tri_w[c];
+        return 2;
+    }
+    if (type == 1) {
+        int4 t = a.tet_v[c];
+        const float* B = a.tet_binv + 9 * c; // row-major 3x3
+        float3 xa = ld3(x, t.x), e1 = ld3(x, t.y) - xa, e2 = ld3(x, t.z) - xa, e3 = ld3(x, t.w) - xa;
+#pragma unroll
+        for (int r = 0; r < 3; r++) dx[r] = (B[r] * e1 + B[3 + r] * e2) + B[6 + r] * e3;
+        *row0 = a.tet_row0 + 3 * c;
+        *w = a.tet_w[c];
+        return 3;
+    }
+    if (type == 2) {
+        float3 s = f3(0.f, 0.f, 0.f);
+        for (int e = a.bend_ptr[c]; e < a.bend_ptr[c + 1]; e++) s = s + a.bend_coef[e] * ld3(x, a.bend_idx[e]);
+        dx[0] = s;
+        *row0 = a.bend_row0 + c;
+        *w = a.bend_w[c];
+        return 1;
+    }
+    dx[0] = ld3(x, a.pin_v[c]);
+    *row0 = a.pin_row0 + c;
+    *w = a.pin_w[c];
+    return 1;
+}
+
+// Stiefel polar factor of F = [y0 y1] (3x2): P = F (F^T F)^{-1/2}, using the
+// closed-form square root of a 2x2 SPD matrix C: sqrt(C) = (C + sI)/t with
+// s = sqrt(det C), t = sqrt(tr C + 2s).
+__device__ inline void prox_tri(float3& y0, float3& y1) {
+    float a = dot3(y0, y0), b = dot3(y0, y1), c = dot3(y1, y1);
+    float s = sqrtf(fmaxf(a * c - b * b, 0.f));
+    float t = sqrtf(a + c + 2.f * s);
+    float m00 = (a + s) / t, m01 = b / t, m11 = (c + s) / t;
+    float det = m00 * m11 - m01 * m01;
+    float i00 = m11 / det, i01 = -m01 / det, i11 = m00 / det;
+    float3 p0 = i00 * y0 + i01 * y1;
+    float3 p1 = i01 * y0 + i11 * y1;
+    y0 = 0.5f * (p0 + y0);
+    y1 = 0.5f * (p1 + y1);
+}
+
+// Closest rotation R to F = [y0 y1 y2]: cyclic Jacobi eigen-decomposition of
+// F^T F (V sorted by descending eigenvalue, det V = +1), U by Gram-Schmidt of
+// F V with u2 = u0 x u1, R = U V^T (det R = +1, so inverted elements map to
+// the nearest rotation).
+__device__ inline void prox_tet(float3& y0, float3& y1, float3& y2) {
+    float F[3][3] = {{y0.x, y1.x, y2.x}, {y0.y, y1.y, y2.y}, {y0.z, y1.z, y2.z}};
+    float A[3][3], V[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) A[i][j] = (F[0][i] * F[0][j] + F[1][i] * F[1][j]) + F[2][i] * F[2][j];
+    for (int sweep = 0; sweep < 6; sweep++) {
+#pragma unroll
+        for (int pq = 0; pq < 3; pq++) {
+            const int p = pq == 2 ? 1 : 0, q = pq == 0 ? 1 : 2;
+            float apq = A[p][q];
+            if (fabsf(apq) < 1e-30f) continue;
+            float theta = (A[q][q] - A[p][p]) / (2.f * apq);
+            float t = (theta >= 0.f ? 1.f : -1.f) / (fabsf(theta) + sqrtf(theta * theta + 1.f));
+            float c = 1.f / sqrtf(t * t + 1.f), s = t * c;
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float akp = A[k][p], akq = A[k][q];
+                A[k][p] = c * akp - s * akq;
+                A[k][q] = s * akp + c * akq;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float apk = A[p][k], aqk = A[q][k];
+                A[p][k] = c * apk - s * aqk;
+                A[q][k] = s * apk + c * aqk;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                float vkp = V[k][p], vkq = V[k][q];
+                V[k][p] = c * vkp - s * vkq;
+                V[k][q] = s * vkp + c * vkq;
+            }
+        }
+    }
+    // sort eigenpairs descending
+    float w[3] = {A[0][0], A[1][1], A[2][2]};
+    int o0 = 0, o1 = 1, o2 = 2;
+    if (w[o0] < w[o1]) { int tq = o0; o0 = o1; o1 = tq; }
+    if (w[o1] < w[o2]) { int tq = o1; o1 = o2; o2 = tq; }
+    if (w[o0] < w[o1]) { int tq = o0; o0 = o1; o1 = tq; }
+    float3 v0 = f3(V[0][o0], V[1][o0], V[2][o0]);
+    float3 v1 = f3(V[0][o1], V[1][o1], V[2][o1]);
+    float3 v2 = f3(V[0][o2], V[1][o2], V[2][o2]);
+    float3 vc = f3(v0.y * v1.z - v0.z * v1.y, v0.z * v1.x - v0.x * v1.z, v0.x * v1.y - v0.y * v1.x);
+    if (dot3(vc, v2) < 0.f) v2 = -1.f * v2;
+    auto Fv = [&](float3 vv) {
+        return f3((F[0][0] * vv.x + F[0][1] * vv.y) + F[0][2] * vv.z, (F[1][0] * vv.x + F[1][1] * vv.y) + F[1][2] * vv.z,
+                  (F[2][0] * vv.x + F[2][1] * vv.y) + F[2][2] * vv.z);
+    };
+    auto unit = [](float3 u) {
+        float q = dot3(u, u);
+        if (q > 0.f) {
+            float l = sqrtf(q);
+            return f3(u.x / l, u.y / l, u.z / l);
+        }
+        return u;
+    };
+    float3 b0 = Fv(v0), b1 = Fv(v1);
+    float3 u0 = unit(b0);
+    float3 u1 = unit(b1 - dot3(u0, b1) * u0);
+    float3 u2 = f3(u0.y * u1.z - u0.z * u1.y, u0.z * u1.x - u0.x * u1.z, u0.x * u1.y - u0.y * u1.x);
+    // R[:, r] = sum_k u_k * v_k[r]
+    float3 r0 = (v0.x * u0 + v1.x * u1) + v2.x * u2;
+    float3 r1 = (v0.y * u0 + v1.y * u1) + v2.y * u2;
+    float3 r2 = (v0.z * u0 + v1.z * u1) + v2.z * u2;
+    y0 = 0.5f * (r0 + y0);
+    y1 = 0.5f * (r1 + y1);
+    y2 = 0.5f * (r2 + y2);
+}
+
+// z = prox(D x + Ue); zprev <- previous z (zero on the first iteration).
+__global__ void __launch_bounds__(BLK) k_local(ElemArgs a, int first_iter, const float* __restrict__ x, const float* __restrict__ Ue,
+                                               float* __restrict__ z, float* __restrict__ zprev) {
+    int c;
+    int type = elem_of(a, &c);
+    if (type < 0) return;
+    float3 y[3];
+    int row0;
+    float w;
+    int nr = elem_dx(a, type, c, x, y, &row0, &w);
+    for (int r = 0; r < nr; r++) {
+        y[r] = y[r] + ld3(Ue, row0 + r);
+        st3(zprev, row0 + r, first_iter ? f3(0.f, 0.f, 0.f) : ld3(z, row0 + r));
+    }
+    if (type == 0) prox_tri(y[0], y[1]);
+    else if (type == 1) prox_tet(y[0], y[1], y[2]);
+    else if (type == 2) y[0] = 0.5f * y[0];
+    else {
+        float4 pp = a.pin_pos[c];
+        y[0] = f3(pp.x, pp.y, pp.z);
+    }
+    for (int r = 0; r < nr; r++) st3(z, row0 + r, y[r]);
+}
+
+// Ue += D x - z; residual sums w^2 |Dx - z|^2 and w^2 |Dx|^2 into part[blk*6 + 0..1].
+__global__ void __launch_bounds__(BLK) k_dual_elem(ElemArgs a, int want_res, const float* __restrict__ x, const float* __restrict__ z,
+                                                   float* __restrict__ Ue, double* __restrict__ part) {
+    int c;
+    int type = elem_of(a, &c);
+    double acc[2] = {0, 0};
+    if (type >= 0) {
+        float3 dx[3];
+        int row0;
+        float w;
+        int nr = elem_dx(a, type, c, x, dx, &row0, &w);
+        for (int r = 0; r < nr; r++) {
+            float3 zz = ld3(z, row0 + r);
+            float3 d = dx[r] - zz;
+            st3(Ue, row0 + r, ld3(Ue, row0 + r) + d);
+            double w2 = (double)w * (double)w;
+            acc[0] += w2 * (((double)d.x * d.x + (double)d.y * d.y) + (double)d.z * d.z);
+            acc[1] += w2 * (((double)dx[r].x * dx[r].x + (double)dx[r].y * dx[r].y) + (double)dx[r].z * dx[r].z);
+        }
+    }
+    if (want_res) block_sum_write<2>(acc, part, 6, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Contacts (reference src/solver/contact_solver.cu:90-431).
+// ---------------------------------------------------------------------------
+__global__ void k_contact_count(int nc, const int4* __restrict__ cv, uint32_t* __restrict__ cnt) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int4 v = cv[c];
+    atomicAdd(&cnt[v.x], 1u);
+    atomicAdd(&cnt[v.y], 1u);
+    atomicAdd(&cnt[v.z], 1u);
+    atomicAdd(&cnt[v.w], 1u);
+}
+
+__global__ void k_contact_fill(int nc, int n, const int4* __restrict__ cv, const uint32_t* __restrict__ vptr,
+                               uint32_t* __restrict__ vfill, uint32_t* __restrict__ vent) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int4 v = cv[c];
+    int vs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (vs[k] < n) {
+            uint32_t pos = vptr[vs[k]] + atomicAdd(&vfill[vs[k]], 1u);
+            vent[pos] = (uint32_t)(4 * c + k);
+        }
+}
+
+// Per-vertex entries sorted ascending (contact order) so gathers sum in a fixed order.
+__global__ void k_contact_sort(int n, const uint32_t* __restrict__ vptr, uint32_t* __restrict__ vent) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t b = vptr[i], e = vptr[i + 1];
+    for (uint32_t a = b + 1; a < e; a++) {
+        uint32_t key = vent[a];
+        uint32_t q = a;
+        while (q > b && vent[q - 1] > key) {
+            vent[q] = vent[q - 1];
+            q--;
+        }
+        vent[q] = key;
+    }
+}
+
+// S_c = eps + sum_i b_i^2 w_i^-1 cnt_i; Gamma_ci = b_i / (S_c w_i); K_c = h^-1 h_cN gamma n; r_c = 0.
+__global__ void k_contact_scc(int nc, int n, float eps, float gam, float hinv, const int4* __restrict__ cv,
+                              const float4* __restrict__ bary, const float4* __restrict__ nrm, const uint32_t* __restrict__ cnt,
+                              const float* __restrict__ wc, const float* __restrict__ wc_inv, float4* __restrict__ G,
+                              float4* __restrict__ K, float4* __restrict__ rc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int4 v = cv[c];
+    float4 b = bary[c];
+    float Sc = eps;
+    Sc += b.x * b.x * wc_inv[v.x] * (float)cnt[v.x];
+    Sc += b.y * b.y * wc_inv[v.y] * (float)cnt[v.y];
+    Sc += b.z * b.z * wc_inv[v.z] * (float)cnt[v.z];
+    Sc += b.w * b.w * wc_inv[v.w] * (float)cnt[v.w];
+    float4 g;
+    g.x = v.x < n ? b.x / (Sc * wc[v.x]) : 0.f;
+    g.y = v.y < n ? b.y / (Sc * wc[v.y]) : 0.f;
+    g.z = v.z < n ? b.z / (Sc * wc[v.z]) : 0.f;
+    g.w = v.w < n ? b.w / (Sc * wc[v.w]) : 0.f;
+    G[c] = g;
+    float4 nn = nrm[c];
+    float s = hinv * nn.w * gam;
+    K[c] = make_float4(s * nn.x, s * nn.y, s * nn.z, 0.f);
+    rc[c] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ inline float3 gather_contacts(int i, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent,
+                                         const float4* __restrict__ G, const float4* __restrict__ src) {
+    float3 s = f3(0.f, 0.f, 0.f);
+    for (uint32_t e = vptr[i]; e < vptr[i + 1]; e++) {
+        uint32_t ent = vent[e];
+        uint32_t c = ent >> 2, k = ent & 3u;
+        float4 g = G[c];
+        float gk = k == 0 ? g.x : (k == 1 ? g.y : (k == 2 ? g.z : g.w));
+        float4 d = src[c];
+        s = s + gk * f3(d.x, d.y, d.z);
+    }
+    return s;
+}
+
+// p = (x - x0 + Uc) / h, then p += Gamma r_c (dynamic vertices).
+__global__ void k_vertex_p(int n, float hinv, int have_contacts, const float* __restrict__ x, const float* __restrict__ x0,
+                           const float* __restrict__ Uc, float* __restrict__ p, const uint32_t* __restrict__ vptr,
+                           const uint32_t* __restrict__ vent, const float4* __restrict__ G, const float4* __restrict__ rc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float3 pi = ((ld3(x, i) - ld3(x0, i)) + ld3(Uc, i));
+    pi = hinv * pi;
+    if (have_contacts) pi = pi + gather_contacts(i, vptr, vent, G, rc);
+    st3(p, i, pi);
+}
+
+// One Jacobi sweep of the friction-cone projection, one lane per contact
+// (formula of ADMMImplCPU::_project_one_contact, admm_impl_cpu.cpp:411-448).
+__global__ void k_contact_du(int nc, float mu, const int4* __restrict__ cv, const float4* __restrict__ bary,
+                             const float4* __restrict__ nrm, const float4* __restrict__ K, const float* __restrict__ p,
+                             float4* __restrict__ rc, float4* __restrict__ du) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    int4 v = cv[c];
+    float4 b = bary[c];
+    float3 u = f3(0.f, 0.f, 0.f);
+    u = u + b.x * ld3(p, v.x);
+    u = u + b.y * ld3(p, v.y);
+    u = u + b.z * ld3(p, v.z);
+    u = u + b.w * ld3(p, v.w);
+    float4 kk = K[c];
+    u = u + f3(kk.x, kk.y, kk.z);
+    float4 r4 = rc[c];
+    float3 r = f3(r4.x, r4.y, r4.z);
+    float4 n4 = nrm[c];
+    float3 nn = f3(n4.x, n4.y, n4.z);
+    float3 us = u - r;
+    float uN = dot3(nn, us);
+    float3 uT = us - uN * nn;
+    if (uN < 0.f) {
+        float tau = sqrtf(dot3(uT, uT));
+        float alpha = -mu * uN;
+        uN = 0.f;
+        if (tau <= alpha) uT = f3(0.f, 0.f, 0.f);
+        else uT = (1.f - alpha / tau) * uT;
+    }
+    float3 unew = uN * nn + uT;
+    float3 d = unew - u;
+    du[c] = make_float4(d.x, d.y, d.z, 0.f);
+    rc[c] = make_float4(r.x + d.x, r.y + d.y, r.z + d.z, 0.f);
+}
+
+__global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent,
+                                const float4* __restrict__ G, const float4* __restrict__ du) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (vptr[i] == vptr[i + 1]) return;
+    st3(p, i, ld3(p, i) + gather_contacts(i, vptr, vent, G, du));
+}
+
+// b = bconst + sum_e coef_e (z - Ue)[row_e] + h^2 wc (h p + x0 - Uc), accumulated in FP64.
+// With want_res, also |sum coef (z - zprev)|^2 and |sum coef z|^2 into part[blk*6 + 4..5].
+__global__ void __launch_bounds__(BLK) k_rhs(int n, float h, int contact, int want_res, const int32_t* __restrict__ dt_ptr,
+                                             const int32_t* __restrict__ dt_row, const float* __restrict__ dt_coef,
+                                             const float* __restrict__ z, const float* __restrict__ zprev, const float* __restrict__ Ue,
+                                             const double* __restrict__ bconst, const float* __restrict__ wch2, const float* __restrict__ p,
+                                             const float* __restrict__ x0, const float* __restrict__ Uc, double* __restrict__ b,
+                                             double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < n) {
+        double s[3] = {0, 0, 0}, s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+        for (int e = dt_ptr[i]; e < dt_ptr[i + 1]; e++) {
+            const int r = dt_row[e];
+            const double cf = dt_coef[e];
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const double zr = z[3 * r + a];
+                s[a] += cf * (zr - (double)Ue[3 * r + a]);
+                if (want_res) {
+                    s1[a] += cf * (zr - (double)zprev[3 * r + a]);
+                    s2[a] += cf * zr;
+                }
+            }
+        }
+        const double wv = contact ? (double)wch2[i] : 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            double bi = bconst[3 * i + a] + s[a];
+            if (wv != 0.0) bi += wv * (((double)h * p[3 * i + a] + x0[3 * i + a]) - (double)Uc[3 * i + a]);
+            b[3 * i + a] = bi;
+        }
+        if (want_res) {
+            acc[0] = (s1[0] * s1[0] + s1[1] * s1[1]) + s1[2] * s1[2];
+            acc[1] = (s2[0] * s2[0] + s2[1] * s2[1]) + s2[2] * s2[2];
+        }
+    }
+    if (want_res) block_sum_write<2>(acc, part, 6, 4);
+}
+
+// Uc += x - x0 - h p; residual sums wc |x - x0 - h p|^2, wc |x - x0|^2 into part[blk*6 + 2..3].
+__global__ void __launch_bounds__(BLK) k_dual_vert(int n, float h, int contact, int want_res, const float* __restrict__ x,
+                                                   const float* __restrict__ x0, const float* __restrict__ p, const float* __restrict__ wch2,
+                                                   float* __restrict__ Uc, double* __restrict__ part, double h2inv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[2] = {0, 0};
+    if (i < n && contact) {
+        float3 dx = ld3(x, i) - ld3(x0, i);
+        float3 d = dx - h * ld3(p, i);
+        st3(Uc, i, ld3(Uc, i) + d);
+        double w = (double)wch2[i] * h2inv;
+        if (want_res && w != 0.0) {
+            acc[0] = w * (((double)d.x * d.x + (double)d.y * d.y) + (double)d.z * d.z);
+            acc[1] = w * (((double)dx.x * dx.x + (double)dx.y * dx.y) + (double)dx.z * dx.z);
+        }
+    }
+    if (want_res) block_sum_write<2>(acc, part, 6, 2);
+}
+
+__global__ void __launch_bounds__(BLK) k_resid_final(const double* __restrict__ part, int nparts, double* __restrict__ out) {
+    double s[6];
+    block_reduce_parts<6>(part, nparts, 6, 0, s);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) out[k] = s[k];
+}
+
+// ---------------------------------------------------------------------------
+// Preconditioned conjugate gradients on A = M + h^2 D^T We^2 D + h^2 Wc + Damp
+// (replaces the fixed-count Jacobi sweeps of cu_jacobi_global,
+// src/solver/jacobi_solver.cu:141-180).  Three right-hand sides (xyz) share
+// each SpMV; FP32 vectors, FP64 dot products.  cg_state layout (two slots s,
+// ping-pong by iteration parity): [s*8 + 0..2] rz, [s*8 + 3..5] done flags;
+// [16..18] bb; [19] iterations.
+// ---------------------------------------------------------------------------
+__device__ inline float3 a_row(int i, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
+                               const float* __restrict__ diag, const float* __restrict__ v) {
+    float3 s = diag[i] * ld3(v, i);
+    for (int e = ptr[i]; e < ptr[i + 1]; e++) s = s + val[e] * ld3(v, col[e]);
+    return s;
+}
+
+// Start of one refinement pass: r = b - A x with A = hi + lo and FP64
+// accumulation (the exact residual of the FP32 iterate), then the FP32 PCG
+// for the correction d starts from d = 0: r32 = r, z = M^-1 r32, p = z.
+// Partials: r32.z (3), |r|^2 (3), |b|^2 (3).
+__global__ void __launch_bounds__(BLK) k_cg_resid64(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                    const float* __restrict__ val, const float* __restrict__ val_lo,
+                                                    const float* __restrict__ diag, const float* __restrict__ diag_lo,
+                                                    const float* __restrict__ dinv, const double* __restrict__ bvec,
+                                                    const float* __restrict__ x, float* __restrict__ r, float* __restrict__ zc,
+                                                    float* __restrict__ pd, float* __restrict__ d, double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const double di = (double)diag[i] + (double)diag_lo[i];
+        double ax[3] = {di * x[3 * i], di * x[3 * i + 1], di * x[3 * i + 2]};
+        for (int e = ptr[i]; e < ptr[i + 1]; e++) {
+            const double a = (double)val[e] + (double)val_lo[e];
+            const int c = col[e];
+            ax[0] += a * x[3 * c];
+            ax[1] += a * x[3 * c + 1];
+            ax[2] += a * x[3 * c + 2];
+        }
+        const float dv = dinv[i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const double bk = bvec[3 * i + k];
+            const double rk = bk - ax[k];
+            const float r32 = (float)rk;
+            const float z32 = dv * r32;
+            r[3 * i + k] = r32;
+            zc[3 * i + k] = z32;
+            pd[3 * i + k] = z32;
+            d[3 * i + k] = 0.f;
+            acc[k] = (double)r32 * z32;
+            acc[3 + k] = rk * rk;
+            acc[6 + k] = bk * bk;
+        }
+    }
+    block_sum_write<9>(acc, part, 9, 0);
+}
+
+// Refinement-pass state: inner stop |r_k| <= tol_in |r_0|; the pass is
+// skipped when the FP64 residual already meets |r_0| <= tol_out |b|.
+__global__ void __launch_bounds__(BLK) k_cg_init_state(const double* __restrict__ part, int nparts, double tol_in2, double tol_out2,
+                                                       double* __restrict__ st) {
+    double s[9];
+    block_reduce_parts<9>(part, nparts, 9, 0, s);
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < 3; k++) {
+            st[k] = s[k];
+            st[16 + k] = tol_in2 * s[3 + k];
+            st[20 + k] = s[6 + k];
+            st[3 + k] = (s[3 + k] <= tol_out2 * s[6 + k] || s[3 + k] == 0.0) ? 1.0 : 0.0;
+        }
+        st[19] = 0;
+    }
+}
+
+__global__ void k_axpy_d(int n, const float* __restrict__ d, float* __restrict__ x) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * n) return;
+    x[i] += d[i];
+}
+
+// q = A pd; partial pq
+__global__ void __launch_bounds__(BLK) k_cg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                 const float* __restrict__ val, const float* __restrict__ diag,
+                                                 const float* __restrict__ pd, float* __restrict__ q, double* __restrict__ part) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[3] = {0, 0, 0};
+    if (i < n) {
+        float3 qi = a_row(i, ptr, col, val, diag, pd);
+        st3(q, i, qi);
+        float3 pi = ld3(pd, i);
+        acc[0] = (double)pi.x * qi.x;
+        acc[1] = (double)pi.y * qi.y;
+        acc[2] = (double)pi.z * qi.z;
+    }
+    block_sum_write<3>(acc, part, 9, 0);
+}
+
+// alpha = rz / pq; x += alpha pd; r -= alpha q; zc = dinv r; partial rz_new, rr
+__global__ void __launch_bounds__(BLK) k_cg_update(int n, int slot, int nparts, const double* __restrict__ st_in,
+                                                   const float* __restrict__ dinv, const float* __restrict__ pd,
+                                                   const float* __restrict__ q, float* __restrict__ x, float* __restrict__ r,
+                                                   float* __restrict__ zc, double* __restrict__ part) {
+    double pq[3];
+    block_reduce_parts<3>(part, nparts, 9, 0, pq);
+    const double* s = st_in + slot * 8;
+    float al[3];
+    for (int k = 0; k < 3; k++) al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        float3 pi = ld3(pd, i), qi = ld3(q, i);
+        float3 xi = ld3(x, i), ri = ld3(r, i);
+        xi = f3(xi.x + al[0] * pi.x, xi.y + al[1] * pi.y, xi.z + al[2] * pi.z);
+        ri = f3(ri.x - al[0] * qi.x, ri.y - al[1] * qi.y, ri.z - al[2] * qi.z);
+        float3 zi = dinv[i] * ri;
+        st3(x, i, xi);
+        st3(r, i, ri);
+        st3(zc, i, zi);
+        acc[0] = (double)ri.x * zi.x;
+        acc[1] = (double)ri.y * zi.y;
+        acc[2] = (double)ri.z * zi.z;
+        acc[3] = (double)ri.x * ri.x;
+        acc[4] = (double)ri.y * ri.y;
+        acc[5] = (double)ri.z * ri.z;
+    }
+    __syncthreads();
+    block_sum_write<6>(acc, part, 9, 3);
+}
+
+// beta = rz_new / rz_old; pd = zc + beta pd; block 0 advances the state slot.
+__global__ void __launch_bounds__(BLK) k_cg_dir(int n, int slot, int nparts, double* __restrict__ st,
+                                                const float* __restrict__ zc, float* __restrict__ pd, const double* __restrict__ part) {
+    double s6[6];
+    block_reduce_parts<6>(part, nparts, 9, 3, s6);
+    const double* so = st + slot * 8;
+    float be[3];
+    double done[3];
+    bool all_done_before = true;
+    for (int k = 0; k < 3; k++) {
+        be[k] = (so[k] == 0.0) ? 0.f : (float)(s6[k] / so[k]);
+        all_done_before = all_done_before && so[3 + k] != 0.0;
+        done[k] = (so[3 + k] != 0.0 || s6[3 + k] <= st[16 + k]) ? 1.0 : 0.0;
+    }
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        float3 zi = ld3(zc, i), pi = ld3(pd, i);
+        st3(pd, i, f3(zi.x + be[0] * pi.x, zi.y + be[1] * pi.y, zi.z + be[2] * pi.z));
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double* sn = st + (slot ^ 1) * 8;
+        for (int k = 0; k < 3; k++) {
+            sn[k] = s6[k];
+            sn[3 + k] = done[k];
+        }
+        if (!all_done_before) st[19] += 1.0;
+    }
+}
+
+} // namespace
+
+// ===========================================================================
+// Host side
+// ===========================================================================
+Backend::Backend(const pa_device_opts* o) {
+    device = o ? o->device : 0;
+    verbose = o ? o->verbose : 0;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    PA_CHECK(e == hipSuccess && count > 0, PA_ERR_NO_DEVICE, "no HIP device visible");
+    PA_CHECK(device >= 0 && device < count, PA_ERR_INVALID, "device ordinal out of range");
+    PA_HIP(hipSetDevice(device));
+    PA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    pa_admm_config_default(&cfg);
+}
+
+Backend::~Backend() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto e : ev_pool) (void)hipEventDestroy(e);
+    // an attached external detector goes back to its own stream
+    if (det && det != own_det.get()) det->stream = det->own_stream;
+    own_det.reset();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Backend::apply_config(const pa_admm_config& c) {
+    PA_CHECK(c.dt > 0, PA_ERR_INVALID, "dt must be positive");
+    PA_CHECK(c.admm_max_iter >= 1, PA_ERR_INVALID, "admm_max_iter must be >= 1");
+    PA_CHECK(c.dcd_interval >= 1, PA_ERR_INVALID, "dcd_interval must be >= 1");
+    PA_CHECK(c.gs_max_iter >= 0, PA_ERR_INVALID, "gs_max_iter must be >= 0");
+    PA_CHECK(c.global_max_iter >= 1, PA_ERR_INVALID, "global_max_iter must be >= 1");
+    PA_CHECK(c.max_contacts >= 1, PA_ERR_INVALID, "max_contacts must be >= 1");
+    PA_CHECK(!initialized, PA_ERR_INVALID, "apply_config must precede init");
+    cfg = c;
+    if (cfg.residual_interval < 1) cfg.residual_interval = 1;
+    have_cfg = true;
+}
+
+void Backend::set_constraints(const pa_constraint_soa& c) {
+    PA_CHECK(!finalized, PA_ERR_INVALID, "constraints already finalized");
+    HostConstraints h;
+    h.n_tri = c.n_tri;
+    h.n_tet = c.n_tet;
+    h.n_bend = c.n_bend;
+    h.n_pin = c.n_pin;
+    PA_CHECK(h.n_tri >= 0 && h.n_tet >= 0 && h.n_bend >= 0 && h.n_pin >= 0, PA_ERR_INVALID, "negative constraint count");
+    h.tri_row0 = c.tri_row0;
+    h.tet_row0 = c.tet_row0;
+    h.bend_row0 = c.bend_row0;
+    h.pin_row0 = c.pin_row0;
+    auto cp_i = [](const int32_t* p, size_t n, std::vector<int32_t>& out) {
+        PA_CHECK(n == 0 || p, PA_ERR_INVALID, "null constraint array");
+        out.assign(p, p + n);
+    };
+    auto cp_d = [](const double* p, size_t n, std::vector<double>& out) {
+        PA_CHECK(n == 0 || p, PA_ERR_INVALID, "null constraint array");
+        out.assign(p, p + n);
+    };
+    cp_i(c.tri_inds, 3 * (size_t)h.n_tri, h.tri_inds);
+    cp_d(c.tri_binv, 4 * (size_t)h.n_tri, h.tri_binv);
+    cp_d(c.tri_w, h.n_tri, h.tri_w);
+    cp_i(c.tet_inds, 4 * (size_t)h.n_tet, h.tet_inds);
+    cp_d(c.tet_binv, 9 * (size_t)h.n_tet, h.tet_binv);
+    cp_d(c.tet_w, h.n_tet, h.tet_w);
+    cp_i(c.bend_ptr, h.n_bend ? (size_t)h.n_bend + 1 : 0, h.bend_ptr);
+    if (h.n_bend == 0) h.bend_ptr = {0};
+    size_t nbe = h.bend_ptr.back();
+    cp_i(c.bend_inds, nbe, h.bend_inds);
+    cp_d(c.bend_coef, nbe, h.bend_coef);
+    cp_d(c.bend_w, h.n_bend, h.bend_w);
+    cp_i(c.pin_inds, h.n_pin, h.pin_inds);
+    cp_d(c.pin_pos, 3 * (size_t)h.n_pin, h.pin_pos);
+    cp_d(c.pin_w, h.n_pin, h.pin_w);
+    hc = std::move(h);
+    have_constraints = true;
+}
+
+void Backend::finalize_constraints() {
+    PA_CHECK(have_constraints, PA_ERR_INVALID, "finalize_constraints before set_constraints");
+    PA_CHECK(m_declared >= 0, PA_ERR_INVALID, "finalize_constraints before set_constraint_dim");
+    // Rows must tile [0, m): tri, tet, bend, pin blocks, each contiguous.
+    struct Blk {
+        int row0, rows;
+    } blks[4] = {{hc.tri_row0, 2 * hc.n_tri}, {hc.tet_row0, 3 * hc.n_tet}, {hc.bend_row0, hc.n_bend}, {hc.pin_row0, hc.n_pin}};
+    std::vector<std::pair<int, int>> iv;
+    for (auto& b : blks)
+        if (b.rows > 0) iv.push_back({b.row0, b.rows});
+    std::sort(iv.begin(), iv.end());
+    int cur = 0;
+    for (auto& q : iv) {
+        PA_CHECK(q.first == cur, PA_ERR_INVALID, "constraint rows must tile [0, m) without gaps or overlap");
+        cur += q.second;
+    }
+    PA_CHECK(cur == m_declared, PA_ERR_INVALID, "constraint dimension does not match the constraints");
+    for (size_t i = 1; i < hc.bend_ptr.size(); i++)
+        PA_CHECK(hc.bend_ptr[i] >= hc.bend_ptr[i - 1], PA_ERR_INVALID, "bend_ptr must be non-decreasing");
+    m = m_declared;
+    finalized = true;
+}
+
+void Backend::init(const double* x0h, const double* massh, int n_all_, int n_dyn, const uint8_t* surface, int n_pins, const int32_t* pins) {
+    PA_CHECK(have_cfg, PA_ERR_INVALID, "init before apply_config");
+    PA_CHECK(finalized, PA_ERR_INVALID, "init before finalize_constraints");
+    PA_CHECK(!initialized, PA_ERR_INVALID, "init called twice");
+    PA_CHECK(n_all_ > 0 && n_dyn > 0 && n_dyn <= n_all_, PA_ERR_INVALID, "bad vertex counts");
+    n_all = n_all_;
+    n = n_dyn;
+    auto chk_v = [&](int v) { PA_CHECK(v >= 0 && v < n, PA_ERR_INVALID, "constraint references a non-dynamic vertex"); };
+    for (int v : hc.tri_inds) chk_v(v);
+    for (int v : hc.tet_inds) chk_v(v);
+    for (int v : hc.bend_inds) chk_v(v);
+    for (int v : hc.pin_inds) chk_v(v);
+    mass_h.assign(massh, massh + n_all);
+    for (int i = 0; i < n; i++) PA_CHECK(mass_h[i] > 0, PA_ERR_INVALID, "dynamic vertex with non-positive mass");
+    surface_h.assign(n_all, 0);
+    if (surface)
+        for (int i = 0; i < n_all; i++) surface_h[i] = surface[i] ? 1 : 0;
+    pinned_h.assign(n_all, 0);
+    for (int i = 0; i < n_pins; i++) {
+        PA_CHECK(pins[i] >= 0 && pins[i] < n_all, PA_ERR_INVALID, "pin out of range");
+        pinned_h[pins[i]] = 1;
+    }
+    assemble_and_upload(x0h);
+    initialized = true;
+}
+
+void Backend::assemble_and_upload(const double* x0h) {
+    const double h = cfg.dt, h2 = h * h;
+    const bool contact = cfg.enable_contact != 0;
+    // ---- per-row D coefficients (row, vertex, G) ----
+    struct Ent {
+        int row, v;
+        double g, w;
+        bool pin;
+    };
+    std::vector<Ent> ents;
+    ents.reserve(6 * (size_t)hc.n_tri + 12 * (size_t)hc.n_tet + hc.bend_inds.size() + hc.n_pin);
+    for (int c = 0; c < hc.n_tri; c++) {
+        const double* B = &hc.tri_binv[4 * c];
+        const int* t = &hc.tri_inds[3 * c];
+        for (int r = 0; r < 2; r++) {
+            int row = hc.tri_row0 + 2 * c + r;
+            ents.push_back({row, t[0], -(B[r] + B[2 + r]), hc.tri_w[c], false});
+            ents.push_back({row, t[1], B[r], hc.tri_w[c], false});
+            ents.push_back({row, t[2], B[2 + r], hc.tri_w[c], false});
+        }
+    }
+    for (int c = 0; c < hc.n_tet; c++) {
+        const double* B = &hc.tet_binv[9 * c];
+        const int* t = &hc.tet_inds[4 * c];
+        for (int r = 0; r < 3; r++) {
+            int row = hc.tet_row0 + 3 * c + r;
+            ents.push_back({row, t[0], -((B[r] + B[3 + r]) + B[6 + r]), hc.tet_w[c], false});
+            for (int q = 0; q < 3; q++) ents.push_back({row, t[q + 1], B[3 * q + r], hc.tet_w[c], false});
+        }
+    }
+    for (int c = 0; c < hc.n_bend; c++)
+        for (int e = hc.bend_ptr[c]; e < hc.bend_ptr[c + 1]; e++)
+            ents.push_back({hc.bend_row0 + c, hc.bend_inds[e], hc.bend_coef[e], hc.bend_w[c], false});
+    for (int c = 0; c < hc.n_pin; c++) ents.push_back({hc.pin_row0 + c, hc.pin_inds[c], 1.0, hc.pin_w[c], true});
+    // group by row (entries of one row are consecutive already)
+    // ---- A = M + h^2 D^T We^2 D + h^2 Wc + h (kL D^T Wd D + kM M);  Damp separately ----
+    struct T3 {
+        int r, c;
+        double v;
+    };
+    std::vector<T3> ta, td;
+    ta.reserve(ents.size() * 8);
+    for (size_t s = 0; s < ents.size();) {
+        size_t e = s;
+        while (e < ents.size() && ents[e].row == ents[s].row) e++;
+        const double w = ents[s].w;
+        for (size_t a = s; a < e; a++)
+            for (size_t bb = s; bb < e; bb++) {
+                double gg = ents[a].g * ents[bb].g;
+                ta.push_back({ents[a].v, ents[bb].v, h2 * w * w * gg});
+                if (!ents[s].pin && cfg.damp_kl != 0) {
+                    double dv = h * cfg.damp_kl * w * gg;
+                    ta.push_back({ents[a].v, ents[bb].v, dv});
+                    td.push_back({ents[a].v, ents[bb].v, dv});
+                }
+            }
+        s = e;
+    }
+    std::vector<float> wch2_h(n_all, 0.f), wc_h(n_all, 1e12f), wcinv_h(n_all, 0.f);
+    for (int i = 0; i < n; i++) {
+        double wci = mass_h[i] / h2 * cfg.w_scale;
+        wc_h[i] = (float)wci;
+        wcinv_h[i] = (float)(1.0 / wci);
+        ta.push_back({i, i, mass_h[i]});
+        if (contact && surface_h[i]) {
+            ta.push_back({i, i, h2 * wci});
+            wch2_h[i] = (float)(h2 * wci);
+        }
+        if (cfg.damp_km != 0) {
+            ta.push_back({i, i, h * cfg.damp_km * mass_h[i]});
+            td.push_back({i, i, h * cfg.damp_km * mass_h[i]});
+        }
+    }
+    auto to_csr = [&](std::vector<T3>& t, std::vector<int32_t>& ptr, std::vector<int32_t>& col, std::vector<double>& val) {
+        std::sort(t.begin(), t.end(), [](const T3& a, const T3& b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+        ptr.assign(n + 1, 0);
+        col.clear();
+        val.clear();
+        for (size_t s = 0; s < t.size();) {
+            size_t e = s;
+            double acc = 0;
+            while (e < t.size() && t[e].r == t[s].r && t[e].c == t[s].c) acc += t[e++].v;
+            col.push_back(t[s].c);
+            val.push_back(acc);
+            ptr[t[s].r + 1]++;
+            s = e;
+        }
+        for (int i = 0; i < n; i++) ptr[i + 1] += ptr[i];
+    };
+    std::vector<int32_t> aptr, acol, dptr, dcol;
+    std::vector<double> aval, dval;
+    to_csr(ta, aptr, acol, aval);
+    to_csr(td, dptr, dcol, dval);
+    // split A into diagonal + off-diagonal
+    std::vector<int32_t> optr(n + 1, 0), ocol;
+    std::vector<float> oval, oval_lo, diag(n, 0.f), diag_lo(n, 0.f), dinv(n, 0.f);
+    for (int i = 0; i < n; i++) {
+        for (int e = aptr[i]; e < aptr[i + 1]; e++) {
+            const float hi = (float)aval[e];
+            const float lo = (float)(aval[e] - (double)hi);
+            if (acol[e] == i) {
+                diag[i] = hi;
+                diag_lo[i] = lo;
+            } else {
+                ocol.push_back(acol[e]);
+                oval.push_back(hi);
+                oval_lo.push_back(lo);
+            }
+        }
+        optr[i + 1] = (int32_t)ocol.size();
+        PA_CHECK(diag[i] > 0, PA_ERR_NOT_SPD, "system matrix has a non-positive diagonal");
+        dinv[i] = 1.f / diag[i];
+    }
+    nnz_a_off = (int64_t)ocol.size();
+    nnz_dm = (int64_t)dcol.size();
+    // D^T We^2 h^2 gather, per vertex sorted by row
+    std::vector<std::array<double, 3>> dt; // (v, row, coef)
+    dt.reserve(ents.size());
+    for (auto& e : ents) dt.push_back({(double)e.v, (double)e.row, h2 * e.w * e.w * e.g});
+    std::sort(dt.begin(), dt.end(), [](const std::array<double, 3>& a, const std::array<double, 3>& b) {
+        return a[0] != b[0] ? a[0] < b[0] : a[1] < b[1];
+    });
+    std::vector<int32_t> dtptr(n + 1, 0), dtrow(dt.size());
+    std::vector<float> dtcoef(dt.size());
+    for (size_t k = 0; k < dt.size(); k++) {
+        dtptr[(int)dt[k][0] + 1]++;
+        dtrow[k] = (int32_t)dt[k][1];
+        dtcoef[k] = (float)dt[k][2];
+    }
+    for (int i = 0; i < n; i++) dtptr[i + 1] += dtptr[i];
+
+    // ---- upload ----
+    hipStream_t s = stream;
+    std::vector<float> xf(3 * (size_t)n_all), mf(n_all);
+    for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)x0h[i];
+    for (int i = 0; i < n_all; i++) mf[i] = (float)mass_h[i];
+    x.alloc(3 * (size_t)n_all);
+    x.upload(xf, s);
+    x0.alloc(3 * (size_t)n_all);
+    v.alloc(3 * (size_t)n_all);
+    fill<float>(v.p, 0.f, 3 * (int64_t)n_all, s);
+    p.alloc(3 * (size_t)n_all);
+    Ue.alloc(3 * (size_t)std::max(m, 1));
+    fill<float>(Ue.p, 0.f, 3 * (int64_t)m, s);
+    Uc.alloc(3 * (size_t)n);
+    fill<float>(Uc.p, 0.f, 3 * (int64_t)n, s);
+    z.alloc(3 * (size_t)std::max(m, 1));
+    fill<float>(z.p, 0.f, 3 * (int64_t)m, s);
+    zprev.alloc(3 * (size_t)std::max(m, 1));
+    fill<float>(zprev.p, 0.f, 3 * (int64_t)m, s);
+    b.alloc(3 * (size_t)n);
+    bconst.alloc(3 * (size_t)n);
+    mass.upload(mf, s);
+    wc.upload(wc_h, s);
+    wc_inv.upload(wcinv_h, s);
+    wch2.upload(wch2_h, s);
+    pinned.upload(pinned_h, s);
+    // constraints
+    {
+        std::vector<int4> tv(hc.n_tri);
+        std::vector<float4> tb(hc.n_tri);
+        std::vector<float> tw(hc.n_tri);
+        for (int c = 0; c < hc.n_tri; c++) {
+            tv[c] = make_int4(hc.tri_inds[3 * c], hc.tri_inds[3 * c + 1], hc.tri_inds[3 * c + 2], 0);
+            tb[c] = make_float4((float)hc.tri_binv[4 * c], (float)hc.tri_binv[4 * c + 1], (float)hc.tri_binv[4 * c + 2],
+                                (float)hc.tri_binv[4 * c + 3]);
+            tw[c] = (float)hc.tri_w[c];
+        }
+        tri_v.upload(tv, s);
+        tri_binv.upload(tb, s);
+        tri_w.upload(tw, s);
+        std::vector<int4> qv(hc.n_tet);
+        std::vector<float> qb(9 * (size_t)hc.n_tet), qw(hc.n_tet);
+        for (int c = 0; c < hc.n_tet; c++) {
+            qv[c] = make_int4(hc.tet_inds[4 * c], hc.tet_inds[4 * c + 1], hc.tet_inds[4 * c + 2], hc.tet_inds[4 * c + 3]);
+            for (int k = 0; k < 9; k++) qb[9 * c + k] = (float)hc.tet_binv[9 * c + k];
+            qw[c] = (float)hc.tet_w[c];
+        }
+        tet_v.upload(qv, s);
+        tet_binv.upload(qb, s);
+        tet_w.upload(qw, s);
+        std::vector<float> bc(hc.bend_coef.begin(), hc.bend_coef.end()), bw(hc.bend_w.begin(), hc.bend_w.end());
+        bend_ptr.upload(hc.bend_ptr, s);
+        bend_idx.upload(hc.bend_inds, s);
+        bend_coef.upload(bc, s);
+        bend_w.upload(bw, s);
+        std::vector<float4> pp(hc.n_pin);
+        std::vector<float> pw(hc.n_pin);
+        for (int c = 0; c < hc.n_pin; c++) {
+            pp[c] = make_float4((float)hc.pin_pos[3 * c], (float)hc.pin_pos[3 * c + 1], (float)hc.pin_pos[3 * c + 2], 0.f);
+            pw[c] = (float)hc.pin_w[c];
+        }
+        pin_v.upload(hc.pin_inds, s);
+        pin_pos.upload(pp, s);
+        pin_w.upload(pw, s);
+    }
+    dt_ptr.upload(dtptr, s);
+    dt_row.upload(dtrow, s);
+    dt_coef.upload(dtcoef, s);
+    a_ptr.upload(optr, s);
+    a_col.upload(ocol, s);
+    a_val.upload(oval, s);
+    a_val_lo.upload(oval_lo, s);
+    a_diag.upload(diag, s);
+    a_diag_lo.upload(diag_lo, s);
+    a_dinv.upload(dinv, s);
+    {
+        std::vector<float> dvf(dval.begin(), dval.end());
+        dm_ptr.upload(dptr, s);
+        dm_col.upload(dcol, s);
+        dm_val.upload(dvf, s);
+    }
+    cg_r.alloc(3 * (size_t)n);
+    cg_z.alloc(3 * (size_t)n);
+    cg_p.alloc(3 * (size_t)n);
+    cg_q.alloc(3 * (size_t)n);
+    cg_d.alloc(3 * (size_t)n);
+    cg_blocks = grid_for(n, BLK);
+    cg_part.alloc(9 * (size_t)cg_blocks);
+    cg_state.alloc(24);
+    int elem_blocks = grid_for(hc.n_tri, BLK) + grid_for(hc.n_tet, BLK) + grid_for(hc.n_bend, BLK) + grid_for(hc.n_pin, BLK);
+    res_blocks = std::max(std::max(elem_blocks, grid_for(n, BLK)), 1);
+    res_part.alloc(6 * (size_t)res_blocks);
+    res_sum.alloc(6);
+    // contact buffers sized by capacity
+    const int cap = cfg.max_contacts;
+    gam.alloc(cap);
+    kc.alloc(cap);
+    rc.alloc(cap);
+    du.alloc(cap);
+    vcnt.alloc(n_all);
+    vptr.alloc((size_t)n + 1);
+    vfill.alloc(n);
+    vent.alloc(4 * (size_t)cap);
+    x_host.assign(x0h, x0h + 3 * (size_t)n_all);
+    v_host.assign(3 * (size_t)n_all, 0.0);
+    PA_HIP(hipStreamSynchronize(s));
+}
+
+void Backend::attach_detector(Detector* d) {
+    PA_CHECK(d, PA_ERR_INVALID, "null detector");
+    if (initialized) PA_CHECK(d->n_all == n_all, PA_ERR_INVALID, "detector vertex count differs from the backend");
+    PA_CHECK(d->c.cap <= cfg.max_contacts, PA_ERR_INVALID, "detector capacity exceeds max_contacts");
+    det = d;
+    det->set_stream(stream);
+}
+
+static ElemArgs make_elem_args(const Backend& B) {
+    ElemArgs a;
+    const HostConstraints& h = B.hc;
+    a.n_tri = h.n_tri;
+    a.n_tet = h.n_tet;
+    a.n_bend = h.n_bend;
+    a.n_pin = h.n_pin;
+    a.tri_row0 = h.tri_row0;
+    a.tet_row0 = h.tet_row0;
+    a.bend_row0 = h.bend_row0;
+    a.pin_row0 = h.pin_row0;
+    a.blk_tri = grid_for(h.n_tri, BLK);
+    a.blk_tet = a.blk_tri + grid_for(h.n_tet, BLK);
+    a.blk_bend = a.blk_tet + grid_for(h.n_bend, BLK);
+    a.tri_v = B.tri_v.p;
+    a.tri_binv = B.tri_binv.p;
+    a.tri_w = B.tri_w.p;
+    a.tet_v = B.tet_v.p;
+    a.tet_binv = B.tet_binv.p;
+    a.tet_w = B.tet_w.p;
+    a.bend_ptr = B.bend_ptr.p;
+    a.bend_idx = B.bend_idx.p;
+    a.bend_coef = B.bend_coef.p;
+    a.bend_w = B.bend_w.p;
+    a.pin_v = B.pin_v.p;
+    a.pin_pos = B.pin_pos.p;
+    a.pin_w = B.pin_w.p;
+    return a;
+}
+
+static int elem_grid(const ElemArgs& a, int n_pin) { return a.blk_bend + grid_for(n_pin, BLK); }
+
+void Backend::local_step(bool first_iter) {
+    ElemArgs a = make_elem_args(*this);
+    int g = elem_grid(a, hc.n_pin);
+    if (g == 0) return;
+    hipLaunchKernelGGL(k_local, dim3(g), dim3(BLK), 0, stream, a, first_iter ? 1 : 0, x.p, Ue.p, z.p, zprev.p);
+    PA_LAUNCH_CHECK();
+}
+
+void Backend::dual_update(bool want_res) {
+    ElemArgs a = make_elem_args(*this);
+    int g = elem_grid(a, hc.n_pin);
+    if (g > 0) {
+        hipLaunchKernelGGL(k_dual_elem, dim3(g), dim3(BLK), 0, stream, a, want_res ? 1 : 0, x.p, z.p, Ue.p, res_part.p);
+        PA_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_dual_vert, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)cfg.dt, cfg.enable_contact, want_res ? 1 : 0,
+                       x.p, x0.p, p.p, wch2.p, Uc.p, res_part.p, 1.0 / (cfg.dt * cfg.dt));
+    PA_LAUNCH_CHECK();
+}
+
+void Backend::contact_setup(float eps, float gamma_, float hinv) {
+    nc = det ? det->n_contacts : 0;
+    fill<uint32_t>(vcnt.p, 0u, n_all, stream);
+    fill<uint32_t>(vfill.p, 0u, n, stream);
+    if (nc > 0)
+        hipLaunchKernelGGL(k_contact_count, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, det->c.v.p, vcnt.p);
+    PA_LAUNCH_CHECK();
+    // per-vertex contact slots over dynamic vertices; static vertices keep
+    // their counts (they enter S_c) but get no slots
+    DBuf<uint32_t>& tmp = scratch.keys_alt;
+    tmp.reserve((size_t)n + 1);
+    PA_HIP(hipMemcpyAsync(tmp.p, vcnt.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, stream));
+    scan_exclusive(tmp.p, vptr.p, n, vptr.p + n, scratch, stream);
+    if (nc > 0) {
+        hipLaunchKernelGGL(k_contact_fill, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, det->c.v.p, vptr.p, vfill.p, vent.p);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_contact_sort, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, vptr.p, vent.p);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_contact_scc, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, eps, gamma_, hinv, det->c.v.p,
+                           det->c.bary.p, det->c.nrm.p, vcnt.p, wc.p, wc_inv.p, gam.p, kc.p, rc.p);
+        PA_LAUNCH_CHECK();
+    }
+}
+
+void Backend::contact_project(bool) {
+    const float hinv = (float)(1.0 / cfg.dt);
+    hipLaunchKernelGGL(k_vertex_p, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, hinv, nc > 0 ? 1 : 0, x.p, x0.p, Uc.p, p.p, vptr.p,
+                       vent.p, gam.p, rc.p);
+    PA_LAUNCH_CHECK();
+    if (nc == 0) return;
+    for (int g = 0; g < cfg.gs_max_iter; g++) contact_sweep_once();
+}
+
+void Backend::contact_sweep_once() {
+    hipLaunchKernelGGL(k_contact_du, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, (float)cfg.mu, det->c.v.p, det->c.bary.p,
+                       det->c.nrm.p, kc.p, p.p, rc.p, du.p);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, p.p, vptr.p, vent.p, gam.p, du.p);
+    PA_LAUNCH_CHECK();
+}
+
+void Backend::rhs(bool want_res, bool) {
+    hipLaunchKernelGGL(k_rhs, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)cfg.dt, cfg.enable_contact, want_res ? 1 : 0,
+                       dt_ptr.p, dt_row.p, dt_coef.p, z.p, zprev.p, Ue.p, bconst.p, wch2.p, p.p, x0.p, Uc.p, b.p, res_part.p);
+    PA_LAUNCH_CHECK();
+}
+
+hipEvent_t Backend::next_event() {
+    if (ev_next >= ev_pool.size()) {
+        hipEvent_t e;
+        PA_HIP(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_next++];
+}
+
+// Mixed-precision refinement: each pass computes the FP64 residual of the
+// FP32 iterate, solves A d = r with FP32 PCG to |r_k| <= global_tol |r_0|,
+// and adds d.  Passes stop early once |r| <= 1e-12 |b| (FP64).  Returns the
+// PCG iterations summed over passes.
+int Backend::global_solve(int max_iter, double tol) {
+    const double tol_in2 = tol * tol, tol_out2 = 1e-24;
+    const int nb = cg_blocks;
+    const int passes = std::max(1, cfg.global_refine);
+    int total_iters = 0;
+    double hs[24];
+    for (int pass = 0; pass < passes; pass++) {
+        hipLaunchKernelGGL(k_cg_resid64, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p, a_diag_lo.p,
+                           a_dinv.p, b.p, x.p, cg_r.p, cg_z.p, cg_p.p, cg_d.p, cg_part.p);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(BLK), 0, stream, cg_part.p, nb, tol_in2, tol_out2, cg_state.p);
+        PA_LAUNCH_CHECK();
+        if (pass > 0) {
+            // a later pass runs only if the FP64 residual still exceeds the floor
+            PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+            PA_HIP(hipStreamSynchronize(stream));
+            if (hs[3] != 0 && hs[4] != 0 && hs[5] != 0) break;
+        }
+        const int chunk = 8;
+        int it = 0;
+        while (it < max_iter) {
+            int stop = std::min(max_iter, it + chunk);
+            for (; it < stop; it++) {
+                const int slot = it & 1;
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (profiling & 1) {
+                    e0 = next_event();
+                    e1 = next_event();
+                    PA_HIP(hipEventRecord(e0, stream));
+                }
+                hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p, cg_p.p, cg_q.p,
+                                   cg_part.p);
+                PA_LAUNCH_CHECK();
+                if (profiling & 1) {
+                    PA_HIP(hipEventRecord(e1, stream));
+                    spmv_marks.push_back({e0, e1});
+                }
+                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, a_dinv.p, cg_p.p, cg_q.p, cg_d.p,
+                                   cg_r.p, cg_z.p, cg_part.p);
+                PA_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
+                PA_LAUNCH_CHECK();
+            }
+            PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+            PA_HIP(hipStreamSynchronize(stream));
+            const double* sl = hs + (it & 1) * 8;
+            if (sl[3] != 0 && sl[4] != 0 && sl[5] != 0) break;
+        }
+        total_iters += (int)hs[19];
+        hipLaunchKernelGGL(k_axpy_d, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, n, cg_d.p, x.p);
+        PA_LAUNCH_CHECK();
+    }
+    return total_iters;
+}
+
+void Backend::residual_read(double* rp, double* rd) {
+    hipLaunchKernelGGL(k_resid_final, dim3(1), dim3(BLK), 0, stream, res_part.p, res_blocks, res_sum.p);
+    PA_LAUNCH_CHECK();
+    double s[6];
+    PA_HIP(hipMemcpyAsync(s, res_sum.p, sizeof(s), hipMemcpyDeviceToHost, stream));
+    PA_HIP(hipStreamSynchronize(stream));
+    *rp = std::sqrt(s[0] + s[2]) / std::max(1.0, std::sqrt(s[1] + s[3]));
+    *rd = std::sqrt(s[4]) / std::max(1.0, std::sqrt(s[5]));
+}
+
+void Backend::mark_begin(int, hipEvent_t* e0) {
+    *e0 = nullptr;
+    if (!(profiling & 2)) return;
+    *e0 = next_event();
+    PA_HIP(hipEventRecord(*e0, stream));
+}
+
+void Backend::mark_end(int phase, hipEvent_t e0) {
+    if (!(profiling & 2)) return;
+    hipEvent_t e1 = next_event();
+    PA_HIP(hipEventRecord(e1, stream));
+    ev_marks.push_back({phase, {e0, e1}});
+}
+
+void Backend::collect_phase_times() {
+    if (!profiling) return;
+    PA_HIP(hipStreamSynchronize(stream));
+    for (int k = 0; k < 6; k++) phase_ms[k] = 0;
+    for (auto& m : ev_marks) {
+        float ms = 0;
+        PA_HIP(hipEventElapsedTime(&ms, m.second.first, m.second.second));
+        phase_ms[m.first] += ms;
+    }
+    for (auto& m : spmv_marks) {
+        float ms = 0;
+        PA_HIP(hipEventElapsedTime(&ms, m.first, m.second));
+        spmv_ms_sum += ms;
+        spmv_launches++;
+    }
+    ev_marks.clear();
+    spmv_marks.clear();
+    ev_next = 0;
+}
+
+// On failure inside the step the positions roll back to the start of the
+// step (velocities are untouched until the epilogue); the duals keep the
+// iterations already applied.
+void Backend::step(pa_step_stats* st) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "step before init");
+    try {
+        step_impl(st);
+    } catch (const Error& err) {
+        if (err.code != PA_ERR_HIP) {
+            (void)hipMemcpyAsync(x.p, x0.p, sizeof(float) * 3 * (size_t)n_all, hipMemcpyDeviceToDevice, stream);
+            (void)hipStreamSynchronize(stream);
+        }
+        if (st) {
+            std::memset(st, 0, sizeof(*st));
+            st->overflow = err.code == PA_ERR_OVERFLOW ? 1 : 0;
+        }
+        throw;
+    }
+}
+
+void Backend::step_impl(pa_step_stats* st) {
+    const bool contact = cfg.enable_contact != 0;
+    PA_CHECK(!contact || det, PA_ERR_INVALID, "contact enabled but no detector attached");
+    auto t0 = std::chrono::steady_clock::now();
+    pa_step_stats S{};
+    const double hd = cfg.dt;
+    const float eps = (float)(1.0 / (hd * hd * cfg.kappa + hd * cfg.beta));
+    const float gam_ = (float)(hd * cfg.kappa / (hd * cfg.kappa + cfg.beta));
+    const float hinv = (float)(1.0 / hd);
+    ev_next = 0;
+    ev_marks.clear();
+    spmv_marks.clear();
+    hipEvent_t e;
+    mark_begin(5, &e);
+    hipLaunchKernelGGL(k_begin_step, dim3(grid_for(n_all, BLK)), dim3(BLK), 0, stream, n_all, n, (float)hd, (float)cfg.g, pinned.p, x.p,
+                       v.p, x0.p, p.p, bconst.p, mass.p);
+    PA_LAUNCH_CHECK();
+    if (nnz_dm > 0) {
+        hipLaunchKernelGGL(k_spmv_add, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, dm_ptr.p, dm_col.p, dm_val.p, x0.p, bconst.p);
+        PA_LAUNCH_CHECK();
+    }
+    if (!cfg.warmstart_ue) fill<float>(Ue.p, 0.f, 3 * (int64_t)m, stream);
+    if (!cfg.warmstart_uc) fill<float>(Uc.p, 0.f, 3 * (int64_t)n, stream);
+    mark_end(5, e);
+    nc = 0;
+    int it = 0;
+    for (; it < cfg.admm_max_iter; it++) {
+        if (contact && it % cfg.dcd_interval == 0) {
+            mark_begin(0, &e);
+            det->detect(x.p);
+            contact_setup(eps, gam_, hinv);
+            S.detections++;
+            mark_end(0, e);
+        }
+        mark_begin(1, &e);
+        local_step(it == 0);
+        mark_end(1, e);
+        if (contact) {
+            mark_begin(2, &e);
+            contact_project(it == 0);
+            mark_end(2, e);
+        }
+        const bool check = cfg.residual_tol > 0 && ((it + 1) % cfg.residual_interval == 0);
+        mark_begin(3, &e);
+        if (check) fill<double>(res_part.p, 0.0, 6 * (int64_t)res_blocks, stream);
+        rhs(check, it == 0);
+        mark_end(3, e);
+        mark_begin(4, &e);
+        S.global_iters += global_solve(cfg.global_max_iter, cfg.global_tol);
+        mark_end(4, e);
+        mark_begin(1, &e);
+        dual_update(check);
+        mark_end(1, e);
+        if (check) {
+            double rp, rd;
+            residual_read(&rp, &rd);
+            S.r_primal = rp;
+            S.r_dual = rd;
+            if (std::max(rp, rd) <= cfg.residual_tol) {
+                S.converged = 1;
+                it++;
+                break;
+            }
+        }
+    }
+    mark_begin(5, &e);
+    hipLaunchKernelGGL(k_end_step, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)hd, x.p, x0.p, v.p);
+    PA_LAUNCH_CHECK();
+    mark_end(5, e);
+    if (sync_to_host) {
+        std::vector<float> xf(3 * (size_t)n_all), vf(3 * (size_t)n_all);
+        x.download(xf.data(), xf.size(), stream);
+        v.download(vf.data(), vf.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < xf.size(); i++) {
+            x_host[i] = xf[i];
+            v_host[i] = vf[i];
+        }
+    } else {
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+    collect_phase_times();
+    S.iterations = it;
+    S.n_contacts = nc;
+    S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = S;
+}
+
+void Backend::reset(const double* x0h) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "reset before init");
+    std::vector<float> xf(3 * (size_t)n_all);
+    for (size_t i = 0; i < xf.size(); i++) xf[i] = (float)x0h[i];
+    x.upload(xf.data(), xf.size(), stream);
+    fill<float>(v.p, 0.f, 3 * (int64_t)n_all, stream);
+    fill<float>(Ue.p, 0.f, 3 * (int64_t)m, stream);
+    fill<float>(Uc.p, 0.f, 3 * (int64_t)n, stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    x_host.assign(x0h, x0h + 3 * (size_t)n_all);
+    std::fill(v_host.begin(), v_host.end(), 0.0);
+}
+
+void Backend::get_state(double* xo, double* vo) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "get_state before init");
+    std::vector<float> xf(3 * (size_t)n_all), vf(3 * (size_t)n_all);
+    x.download(xf.data(), xf.size(), stream);
+    v.download(vf.data(), vf.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    for (size_t i = 0; i < xf.size(); i++) {
+        if (xo) xo[i] = xf[i];
+        if (vo) vo[i] = vf[i];
+    }
+}
+
+void Backend::set_state(const double* xi, const double* vi) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "set_state before init");
+    std::vector<float> f(3 * (size_t)n_all);
+    if (xi) {
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)xi[i];
+        x.upload(f.data(), f.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+    if (vi) {
+        for (size_t i = 0; i < f.size(); i++) f[i] = (float)vi[i];
+        v.upload(f.data(), f.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+}
+
+void Backend::get_duals(double* ue, double* uc) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "get_duals before init");
+    std::vector<float> a(3 * (size_t)m), c(3 * (size_t)n);
+    Ue.download(a.data(), a.size(), stream);
+    Uc.download(c.data(), c.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    if (ue)
+        for (size_t i = 0; i < a.size(); i++) ue[i] = a[i];
+    if (uc)
+        for (size_t i = 0; i < c.size(); i++) uc[i] = c[i];
+}
+
+void Backend::set_duals(const double* ue, const double* uc) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "set_duals before init");
+    if (ue) {
+        std::vector<float> a(ue, ue + 3 * (size_t)m);
+        Ue.upload(a.data(), a.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+    if (uc) {
+        std::vector<float> c(uc, uc + 3 * (size_t)n);
+        Uc.upload(c.data(), c.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+}
+
+void Backend::set_static_state(const double* xs, const double* vs) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "set_static_state before init");
+    const size_t ns = 3 * (size_t)(n_all - n);
+    if (ns == 0) return;
+    std::vector<float> f(ns);
+    if (xs) {
+        for (size_t i = 0; i < ns; i++) f[i] = (float)xs[i];
+        PA_HIP(hipMemcpyAsync(x.p + 3 * (size_t)n, f.data(), ns * sizeof(float), hipMemcpyHostToDevice, stream));
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+    if (vs) {
+        for (size_t i = 0; i < ns; i++) f[i] = (float)vs[i];
+        PA_HIP(hipMemcpyAsync(v.p + 3 * (size_t)n, f.data(), ns * sizeof(float), hipMemcpyHostToDevice, stream));
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+}
+
+} // namespace pa

@@ -1,0 +1,40 @@
+"""Shared builders for parity tests: scenes in contact-rich states."""
+from __future__ import annotations
+
+import numpy as np
+
+
+def drape_on_obstacle(scene, dhat: float, frac: float = 0.4) -> np.ndarray:
+    """Positions with the cloth lowered so its lowest point sits frac*dhat above the
+    highest static vertex below it (a contact patch on top of the obstacle)."""
+    m = scene.mesh()
+    x = m["x"].copy()
+    n = m["n_dyn"]
+    top = x[n:, 1].max()
+    x[:n, 1] += (top + frac * dhat) - x[:n, 1].min()
+    return x
+
+
+def folded_cloth(side: int, spacing: float, gap: float, jitter: float, seed: int = 0):
+    """A side x side cloth folded in half along x: the upper half lies `gap` above
+    the lower half (dense self-contact).  Returns (x, tris) as a raw mesh."""
+    import parallel_admm_amd as pa
+
+    rng = np.random.default_rng(seed)
+    sc = pa.Scene.generate(4, side=side, radius=0.05, drop=0.3)
+    m = sc.mesh()
+    n = m["n_dyn"]
+    x = m["x"].copy()
+    xs = x[:n]
+    xmid = 0.5 * (xs[:, 0].min() + xs[:, 0].max())
+    y0 = xs[:, 1].min()
+    upper = xs[:, 0] > xmid
+    xs[upper, 0] = 2 * xmid - xs[upper, 0]
+    xs[upper, 1] = y0 + gap
+    xs += jitter * rng.standard_normal(xs.shape)
+    x[:n] = xs
+    return x, sc
+
+
+def rel_err(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))

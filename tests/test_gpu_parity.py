@@ -1,0 +1,246 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Bars: contact sets bit-identical (indices, order) with float fields equal to
+the oracle's FP64 values rounded to FP32; elastic prox, contact projection and
+the global solve within FP32 tolerances stated per test; one time step from a
+common state within 1e-5 relative to the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import drape_on_obstacle, folded_cloth, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle(mesh, cfg, **kw):
+    from oracle.pyoracle import Oracle
+
+    return Oracle(mesh, cfg, **kw)
+
+
+def _detector(pa, scene, cfg):
+    return pa.CollisionDetector.for_scene(scene, capacity=cfg.max_contacts, thickness=cfg.thickness)
+
+
+def _check_contacts_equal(gi, gr, oi, orr):
+    assert gi.shape == oi.shape, f"contact count {gi.shape[0]} != oracle {oi.shape[0]}"
+    np.testing.assert_array_equal(gi, oi)
+    # float fields: FP64 computed identically, rounded to FP32 on the device
+    np.testing.assert_array_equal(gr[:, :8], orr[:, :8].astype(np.float32).astype(np.float64))
+    np.testing.assert_array_equal(gr[:, 8], orr[:, 8])
+
+
+@pytest.mark.parametrize("kind,name", [(1, "cloth32"), (2, "tetbox")])
+def test_detector_matches_oracle_on_obstacle(pa, kind, name):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for(name)
+    sc = pa.Scene.generate(kind)
+    x = drape_on_obstacle(sc, cfg.thickness).astype(np.float32).astype(np.float64)
+    det = _detector(pa, sc, cfg)
+    n = det.detect(x)
+    gi, gr = det.contacts()
+    o = _oracle(sc.mesh(), cfg)
+    no = o.detect(x, brute_force=True)
+    oi, orr = o.contacts()
+    print(f"{name}: {n} contacts (oracle {no})")
+    assert n > 0
+    _check_contacts_equal(gi, gr, oi, orr)
+
+
+@pytest.mark.parametrize("jitter", [0.0, 0.15])
+def test_detector_matches_oracle_folded_cloth(pa, jitter):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    x, sc = folded_cloth(side=24, spacing=0.01, gap=0.5 * cfg.thickness, jitter=jitter * cfg.thickness, seed=3)
+    x = x.astype(np.float32).astype(np.float64)
+    det = _detector(pa, sc, cfg)
+    n = det.detect(x)
+    gi, gr = det.contacts()
+    o = _oracle(sc.mesh(), cfg)
+    o.detect(x, brute_force=True)
+    oi, orr = o.contacts()
+    print(f"folded cloth jitter={jitter}: {n} contacts, PT {np.sum(gi[:, 0] == 0)}, EE {np.sum(gi[:, 0] == 1)}")
+    assert n > 100
+    _check_contacts_equal(gi, gr, oi, orr)
+
+
+def test_detector_interface(pa):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    sc = pa.Scene.generate(1)
+    x = drape_on_obstacle(sc, cfg.thickness)
+    det = pa.CollisionDetector.for_scene(sc, capacity=4096, thickness=cfg.thickness)
+    n = det.detect(x)
+    assert det.contact_count() == n and det.max_slots() == 4096 and det.peak_contact_count() >= n
+    pts, nrm = det.points(), det.normals()
+    assert pts.shape == (4096, 3) and nrm.shape == (4096, 3)
+    assert np.all(pts[n:] == 0) and np.all(nrm[n:] == 0)
+    np.testing.assert_allclose(np.linalg.norm(nrm[:n], axis=1), 1.0, atol=1e-6)
+    det.clear()
+    assert det.contact_count() == 0
+
+
+def test_detector_overflow_reports_error(pa):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    x, sc = folded_cloth(side=16, spacing=0.01, gap=0.5 * cfg.thickness, jitter=0.0)
+    det = pa.CollisionDetector.for_scene(sc, capacity=8, thickness=cfg.thickness)
+    with pytest.raises(pa.PAError) as e:
+        det.detect(x)
+    assert e.value.code == pa.PA_ERR_OVERFLOW
+
+
+def test_elastic_prox_matches_oracle(pa):
+    """z = prox(D x + Ue): tri (Stiefel polar), tet (closest rotation), bend, pin.
+    Tolerance 2e-5 relative per block (FP32 vs FP64)."""
+    from parallel_admm_amd.configs import config_for
+
+    rng = np.random.default_rng(7)
+    for kind, name in ((1, "cloth32"), (2, "tetbox")):
+        cfg = config_for(name)
+        sc = pa.Scene.generate(kind)
+        m = sc.mesh()
+        b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+        x = m["x"] + 0.002 * rng.standard_normal(m["x"].shape)
+        x = x.astype(np.float32).astype(np.float64)
+        ue = (0.05 * rng.standard_normal((sc.info["m_rows"], 3))).astype(np.float32).astype(np.float64)
+        zg = b.project_elastic(x, ue)
+        zo = _oracle(m, cfg).project_elastic(x, ue)
+        err = np.abs(zg - zo).max() / np.abs(zo).max()
+        print(f"{name}: max |z_gpu - z_oracle| / max|z| = {err:.2e}")
+        assert err < 2e-5
+
+
+def test_contact_projection_matches_oracle(pa):
+    """Jacobi-weighted friction-cone sweeps on the folded-cloth contact set.
+    Tolerance 1e-4 relative (FP32 accumulation over gs_max_iter sweeps)."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    x, sc = folded_cloth(side=16, spacing=0.01, gap=0.5 * cfg.thickness, jitter=0.1 * cfg.thickness, seed=1)
+    x = x.astype(np.float32).astype(np.float64)
+    m = sc.mesh()
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    b.set_state(x=x)
+    # detection through the backend's own detector
+    rng = np.random.default_rng(2)
+    p = (0.5 * rng.standard_normal(x.shape)).astype(np.float32).astype(np.float64)
+    p[m["n_dyn"]:] = 0
+    det = pa.CollisionDetector.for_scene(sc, capacity=cfg.max_contacts, thickness=cfg.thickness)
+    nc = det.detect(x)
+    b.attach_detector(det)
+    pg, rg = b.project_contacts(p, nc)
+    o = _oracle(m, cfg)
+    o.detect(x)
+    po, ro = o.project_contacts(p)
+    assert nc > 50
+    e1 = np.abs(pg - po).max() / np.abs(po).max()
+    e2 = np.abs(rg - ro).max() / max(np.abs(ro).max(), 1e-12)
+    print(f"{nc} contacts: p err {e1:.2e}, r_c err {e2:.2e}")
+    assert e1 < 1e-4 and e2 < 1e-4
+
+
+def test_global_solve_matches_llt(pa):
+    """Global solve (FP64 residual + FP32 PCG correction, two refinement passes)
+    against the oracle's sparse LLT: relative error <= 1e-6."""
+    from parallel_admm_amd.configs import config_for
+
+    rng = np.random.default_rng(5)
+    for kind, name in ((1, "cloth32"), (2, "tetbox")):
+        cfg = config_for(name)
+        cfg.global_tol = 1e-5
+        cfg.global_refine = 2
+        cfg.global_max_iter = 2000
+        sc = pa.Scene.generate(kind)
+        m = sc.mesh()
+        b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+        o = _oracle(m, cfg)
+        n = sc.info["n_dyn"]
+        xs = m["x"][:n] + 0.01 * rng.standard_normal((n, 3))
+        A = o.matrix(1)
+        rhs = A @ xs
+        xo = o.solve(rhs)
+        xg, it = b.global_solve(rhs, m["x"][:n])
+        err = rel_err(xg, xo)
+        print(f"{name}: PCG {it} iterations, rel err vs LLT {err:.2e}")
+        assert err < 1e-6
+
+
+STEP_CASES = [
+    # (label, config, scene kind, generator args, frames, start state)
+    ("cloth32-free", "cloth32", 1, {}, 4, "rest"),
+    ("cloth32-drape", "cloth32", 4, dict(side=32, radius=0.2, drop=0.002), 6, "rest"),
+    ("tetbox-ground", "tetbox", 2, {}, 4, "drape"),
+]
+
+
+@pytest.mark.parametrize("label,name,kind,gen,frames,start", STEP_CASES, ids=[c[0] for c in STEP_CASES])
+def test_step_matches_fp64_oracle(pa, label, name, kind, gen, frames, start):
+    """One time step from a common state (x, v, Ue, Uc), per frame, within 1e-5
+    relative (||x_gpu - x_oracle|| / ||x_oracle||), with the workload's default
+    solver settings."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for(name)
+    sc = pa.Scene.generate(kind, **gen)
+    m = sc.mesh()
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    if start == "drape":
+        b.set_state(x=drape_on_obstacle(sc, cfg.thickness, frac=2.0))
+    o = _oracle(m, cfg)
+    worst = 0.0
+    contacts = 0
+    for f in range(frames):
+        x, v = b.get_state()
+        ue, uc = b.get_duals()
+        o.set_state(x, v)
+        o.set_duals(ue, uc)
+        st = b.step()
+        so = o.step()
+        xg, vg = b.get_state()
+        xo, vo = o.state()
+        e = rel_err(xg, xo)
+        worst = max(worst, e)
+        contacts += st["n_contacts"]
+        print(f"{label} frame {f}: contacts gpu {st['n_contacts']} oracle {so['n_contacts']}, "
+              f"PCG iters {st['global_iters']}, rel err {e:.2e}")
+    assert worst < 1e-5
+    if start == "drape" or "drape" in label:
+        assert contacts > 0
+
+
+def test_step_deterministic(pa):
+    """Two runs from the same state are bitwise identical."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    sc = pa.Scene.generate(1)
+    x0 = drape_on_obstacle(sc, cfg.thickness, frac=2.0)
+    out = []
+    for _ in range(2):
+        b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+        b.set_state(x=x0)
+        for _ in range(3):
+            b.step()
+        out.append(b.get_state()[0])
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_step_overflow_rolls_back(pa):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.max_contacts = 4
+    x, sc = folded_cloth(side=16, spacing=0.01, gap=0.5 * cfg.thickness, jitter=0.0)
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    b.set_state(x=x)
+    before = b.get_state()[0]
+    with pytest.raises(pa.PAError) as e:
+        b.step()
+    assert e.value.code == pa.PA_ERR_OVERFLOW
+    np.testing.assert_array_equal(b.get_state()[0], before)
