@@ -1,0 +1,150 @@
+"""Pins the oracle's geometric kernels against closed forms, first-order
+optimality (projection onto a convex set) and brute-force minimisation, and its
+prox operators against numpy SVD polar decompositions.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as orc
+
+
+def _grid_min_pt(p, t0, t1, t2, n=300):
+    s, t = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    m = (s + t) <= 1.0
+    s, t = s[m], t[m]
+    pts = t0 + s[:, None] * (t1 - t0) + t[:, None] * (t2 - t0)
+    return np.min(np.linalg.norm(pts - p, axis=1))
+
+
+def test_dist_pt_closed_forms():
+    t0, t1, t2 = np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, 1, 0])
+    # above the interior: distance = height, closest = foot point
+    d = orc.dist_pt(np.array([0.2, 0.3, 0.7]), t0, t1, t2)
+    assert d[0] == pytest.approx(0.7, abs=1e-15)
+    np.testing.assert_allclose(d[2:], [0.5, 0.2, 0.3], atol=1e-15)
+    # vertex region
+    d = orc.dist_pt(np.array([-1.0, -2.0, 2.0]), t0, t1, t2)
+    assert d[0] == pytest.approx(3.0, abs=1e-14)
+    np.testing.assert_allclose(d[2:], [1, 0, 0], atol=0)
+    # edge region (hypotenuse), closest point (0.5, 0.5, 0)
+    d = orc.dist_pt(np.array([1.0, 1.0, 0.0]), t0, t1, t2)
+    assert d[0] == pytest.approx(np.sqrt(0.5), abs=1e-14)
+    np.testing.assert_allclose(d[2:], [0.0, 0.5, 0.5], atol=1e-15)
+
+
+def test_dist_pt_optimality_and_brute_force():
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        t0, t1, t2, p = rng.standard_normal((4, 3))
+        p = p * rng.uniform(0.1, 3.0)
+        d = orc.dist_pt(p, t0, t1, t2)
+        b = d[2:]
+        assert np.all(b >= -1e-12) and abs(b.sum() - 1) < 1e-12
+        c = b[0] * t0 + b[1] * t1 + b[2] * t2
+        assert d[0] == pytest.approx(np.linalg.norm(p - c), rel=1e-12, abs=1e-14)
+        # projection onto a convex set: <p - c, v - c> <= 0 for every vertex v
+        for v in (t0, t1, t2):
+            assert np.dot(p - c, v - c) <= 1e-10 * (1 + np.linalg.norm(p - c) * np.linalg.norm(v - c))
+        # brute force (grid over the simplex) bounds the minimum from above
+        g = _grid_min_pt(p, t0, t1, t2)
+        h = max(np.linalg.norm(t1 - t0), np.linalg.norm(t2 - t0)) / 299
+        assert d[0] <= g + 1e-12 and d[0] >= g - 2 * h
+
+
+def _grid_min_ee(p0, p1, q0, q1, n=400):
+    s = np.linspace(0, 1, n)
+    P = p0 + s[:, None] * (p1 - p0)
+    Q = q0 + s[:, None] * (q1 - q0)
+    return np.min(np.linalg.norm(P[:, None, :] - Q[None, :, :], axis=2))
+
+
+def test_dist_ee_closed_forms():
+    # perpendicular crossing segments separated along z
+    d = orc.dist_ee(np.array([-1.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, -1, 0.5]), np.array([0.0, 1, 0.5]))
+    assert d[0] == pytest.approx(0.5, abs=1e-15)
+    np.testing.assert_allclose(d[2:], [0.5, 0.5], atol=1e-15)
+    # parallel segments, overlapping in x, 0.3 apart
+    d = orc.dist_ee(np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.5, 0.3, 0]), np.array([2.0, 0.3, 0]))
+    assert d[0] == pytest.approx(0.3, abs=1e-15)
+    # endpoint to endpoint
+    d = orc.dist_ee(np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([2.0, 1, 0]), np.array([3.0, 2, 0]))
+    assert d[0] == pytest.approx(np.sqrt(2.0), abs=1e-14)
+    np.testing.assert_allclose(d[2:], [1.0, 0.0], atol=0)
+
+
+def test_dist_ee_optimality_and_brute_force():
+    rng = np.random.default_rng(1)
+    for k in range(300):
+        p0, p1, q0, q1 = rng.standard_normal((4, 3))
+        if k % 5 == 0:  # nearly parallel pairs exercise the parallel branch
+            q1 = q0 + (p1 - p0) * rng.uniform(0.2, 2) + 1e-9 * rng.standard_normal(3)
+        d = orc.dist_ee(p0, p1, q0, q1)
+        s, t = d[2], d[3]
+        assert 0 <= s <= 1 and 0 <= t <= 1
+        c0, c1 = p0 + s * (p1 - p0), q0 + t * (q1 - q0)
+        assert d[0] == pytest.approx(np.linalg.norm(c0 - c1), rel=1e-12, abs=1e-14)
+        g = _grid_min_ee(p0, p1, q0, q1)
+        h = max(np.linalg.norm(p1 - p0), np.linalg.norm(q1 - q0)) / 399
+        # nearly parallel pairs: the interior solution divides by det ~ 1e-18,
+        # so the minimiser is only accurate to about sqrt(eps) in (s, t)
+        slack = 1e-7 if k % 5 == 0 else 1e-12
+        assert d[0] <= g + slack and d[0] >= g - 2 * h
+
+
+def _polar_stiefel_svd(F):
+    U, _, Vt = np.linalg.svd(F, full_matrices=False)
+    return U @ Vt
+
+
+def _closest_rotation_svd(F):
+    U, _, Vt = np.linalg.svd(F)
+    D = np.diag([1.0, 1.0, np.sign(np.linalg.det(U @ Vt))])
+    return U @ D @ Vt
+
+
+def test_prox_tri_is_half_sum_with_stiefel_polar():
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        F = rng.standard_normal((3, 2))
+        z = orc.prox_tri(F.T.copy())
+        P = 2 * z.T - F
+        np.testing.assert_allclose(P, _polar_stiefel_svd(F), atol=1e-10)
+        np.testing.assert_allclose(P.T @ P, np.eye(2), atol=1e-10)
+    # rest state: F with orthonormal columns is a fixed point
+    Q, _ = np.linalg.qr(rng.standard_normal((3, 2)))
+    np.testing.assert_allclose(orc.prox_tri(Q.T.copy()).T, Q, atol=1e-14)
+
+
+def test_prox_tet_is_half_sum_with_closest_rotation():
+    rng = np.random.default_rng(3)
+    for k in range(200):
+        F = rng.standard_normal((3, 3))
+        if k % 4 == 0:
+            F[:, 2] *= -1  # inverted element
+        z = orc.prox_tet(F.T.copy())
+        R = 2 * z.T - F
+        np.testing.assert_allclose(R, _closest_rotation_svd(F), atol=1e-9)
+        assert np.linalg.det(R) == pytest.approx(1.0, abs=1e-10)
+    # F = rotation * diag(stretch) -> R = rotation
+    a = 0.7
+    Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    F = Rz @ np.diag([1.3, 0.8, 1.1])
+    np.testing.assert_allclose(2 * orc.prox_tet(F.T.copy()).T - F, Rz, atol=1e-12)
+    np.testing.assert_allclose(orc.prox_tet(np.eye(3)), np.eye(3), atol=1e-15)
+
+
+def test_llt_matches_dense_solve():
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(4)
+    n = 30
+    # 2-D grid Laplacian + mass: SPD, the structure of the cloth system matrix
+    I = sp.identity(n)
+    T = sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    A = (sp.kron(I, T) + sp.kron(T, I) + 1e-3 * sp.identity(n * n)).tocsr()
+    b = rng.standard_normal(n * n)
+    x, nnz_l = orc.llt_solve(A, b)
+    xd = np.linalg.solve(A.toarray(), b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-11
+    assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) < 1e-12
+    # nested dissection keeps the factor far below the dense n^2/2
+    assert nnz_l < 0.2 * (n * n) ** 2 / 2

@@ -1,0 +1,160 @@
+"""Pins the oracle's time step with closed forms and invariants: implicit-Euler
+free fall, the g = 0 rest state, momentum conservation by the internal forces,
+pinned vertices, the contact set against the all-pairs loop, and one contact's
+projection worked out by hand.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle.pyoracle import Oracle
+
+
+def _cfg(**kw):
+    from parallel_admm_amd import AdmmConfig
+
+    c = AdmmConfig(dt=1.0 / 60, g=9.81, admm_max_iter=10, dcd_interval=5, gs_max_iter=10, mu=0.3,
+                   thickness=1e-3, damp_kl=1e-4, w_scale=50.0)
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def _small_cloth(pa, side=8, k_stretch=1e3, k_bend=1e-2, pins=()):
+    sc = pa.Scene.generate(4, side=side, radius=0.05, drop=0.3)
+    m = sc.mesh()
+    n = m["n_dyn"]
+    m["tri_k"][:] = np.where(np.arange(m["tri_k"].size) < 2 * (side - 1) ** 2, k_stretch, 0.0)
+    m["bend_k"][:n] = k_bend
+    m["pins"] = np.array(pins, np.int32)
+    return m
+
+
+def test_free_fall_is_implicit_euler(pa):
+    """No stiffness and no contact: A = M and every step is x += h (v - g h e_y)."""
+    m = _small_cloth(pa, k_stretch=0.0, k_bend=0.0)
+    o = Oracle(m, _cfg(enable_contact=False))
+    x0, _ = o.state()
+    h, g = 1.0 / 60, 9.81
+    v = np.zeros_like(x0)
+    x = x0.copy()
+    n = m["n_dyn"]
+    for _ in range(5):
+        o.step()
+        v[:n, 1] -= g * h
+        x[:n] += h * v[:n]
+    xo, vo = o.state()
+    np.testing.assert_allclose(xo, x, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(vo[:n], v[:n], rtol=0, atol=1e-11)
+
+
+def test_rest_state_is_stationary_without_gravity(pa):
+    m = _small_cloth(pa)
+    o = Oracle(m, _cfg(g=0.0, enable_contact=False))
+    x0, _ = o.state()
+    for _ in range(3):
+        o.step()
+    np.testing.assert_allclose(o.state()[0], x0, atol=1e-12)
+
+
+def test_internal_forces_conserve_momentum(pa):
+    """g = 0, no pins, no contact: elastic + damping rows annihilate translations,
+    so sum(m v) is unchanged by a step from any velocity field."""
+    m = _small_cloth(pa)
+    o = Oracle(m, _cfg(g=0.0, enable_contact=False, admm_max_iter=15))
+    rng = np.random.default_rng(0)
+    x, _ = o.state()
+    v = np.zeros_like(x)
+    n = m["n_dyn"]
+    v[:n] = 0.3 * rng.standard_normal((n, 3))
+    o.set_state(x, v)
+    mass = o.mass()[:n, None]
+    p0 = (mass * v[:n]).sum(axis=0)
+    for _ in range(3):
+        o.step()
+    p1 = (mass * o.state()[1][:n]).sum(axis=0)
+    np.testing.assert_allclose(p1, p0, atol=1e-12)
+    # and the step did deform the cloth (non-trivial dynamics)
+    assert np.abs(o.state()[0][:n] - x[:n]).max() > 1e-3
+
+
+def test_pinned_vertices_hold(pa):
+    sc = pa.Scene.generate(1)
+    o = Oracle(sc.mesh(), _cfg(admm_max_iter=20))
+    x0, _ = o.state()
+    for _ in range(5):
+        o.step()
+    x, _ = o.state()
+    pins = sc.mesh()["pins"]
+    free = np.setdiff1d(np.arange(1024), pins)
+    assert np.abs(x[pins] - x0[pins]).max() < 2e-4
+    assert np.abs(x[free, 1] - x0[free, 1]).max() > 1e-2
+
+
+def test_box_prefilter_equals_all_pairs(pa):
+    import sys, os
+    sys.path.insert(0, os.path.dirname(__file__))
+    from helpers import folded_cloth
+
+    x, sc = folded_cloth(side=14, spacing=0.01, gap=0.5e-3, jitter=0.15e-3, seed=4)
+    o = Oracle(sc.mesh(), _cfg())
+    na = o.detect(x, brute_force=True)
+    ia, ra = o.contacts()
+    nb = o.detect(x, brute_force=False)
+    ib, rb = o.contacts()
+    assert na == nb and na > 100
+    np.testing.assert_array_equal(ia, ib)
+    np.testing.assert_array_equal(ra, rb)
+    # canonical order: PT by (vertex, face) then EE by (i, j); no shared vertices in EE
+    pt, ee = ia[ia[:, 0] == 0], ia[ia[:, 0] == 1]
+    assert np.all(ia[: len(pt), 0] == 0)
+    assert np.all(np.diff(pt[:, 1] * 10**7 + pt[:, 2]) > 0)
+    assert np.all(np.diff(ee[:, 1] * 10**7 + ee[:, 2]) > 0) and np.all(ee[:, 1] < ee[:, 2])
+    assert all(len({a, b, c, d}) == 4 for a, b, c, d in ee[:, 3:7])
+    # h_cN = n.(c0 - c1) - dhat with n.(c0 - c1) = dist >= 0 for PT interior contacts
+    assert np.all(ra[:, 7] >= -1e-3 - 1e-15) and np.all(ra[:, 7] <= 1e-15)
+
+
+def test_single_contact_projection_by_hand(pa):
+    """One PT contact (dynamic vertex above a large static triangle), one sweep:
+    S_c = eps + w^-1, Gamma = 1 / (S_c w), K = h^-1 h_cN gamma n,
+    u = p_v + K; the cone projection of u gives du; p_v += Gamma du."""
+    h, kappa, beta, mu, dhat, ws = 1.0 / 60, 1e6, 1e5, 0.3, 1e-3, 50.0
+    # dynamic triangle: vertex 0 is 0.4 mm above the static triangle, the others 5 cm up
+    x = np.array([[0.0, 0.0004, 0.0], [0.01, 0.05, 0.0], [0.0, 0.05, 0.01],
+                  [-1.0, 0.0, -1.0], [1.0, 0.0, -1.0], [0.0, 0.0, 1.0]])
+    mesh = dict(x=x, n_dyn=3, density=np.array([1.0, 1.0, 1.0, 1e12, 1e12, 1e12]),
+                tris=np.array([[0, 1, 2], [3, 5, 4]], np.int32), tri_k=np.array([1e3, 0.0]),
+                bend_k=np.zeros(6), tets=np.zeros((0, 4), np.int32), tet_k=np.zeros(0),
+                tet_boundary=np.zeros((0, 3), np.int32), pins=np.zeros(0, np.int32))
+    o = Oracle(mesh, _cfg(mu=mu, kappa=kappa, beta=beta, thickness=dhat, w_scale=ws, gs_max_iter=1))
+    nc = o.detect(x)
+    ints, reals = o.contacts()
+    assert nc == 1 and ints[0, 0] == 0 and ints[0, 3] == 0
+    nrm = reals[0, 4:7]
+    np.testing.assert_allclose(np.abs(nrm), [0, 1, 0], atol=1e-15)
+    hcn = reals[0, 7]
+    assert hcn == pytest.approx(0.0004 - dhat, abs=1e-15)
+    w = o.mass()[0] / h ** 2 * ws
+    eps = 1.0 / (h * h * kappa + h * beta)
+    gam = h * kappa / (h * kappa + beta)
+    Sc = eps + 1.0 / w
+    G = 1.0 / (Sc * w)
+    K = hcn * gam / h * nrm
+    for pv, case in ((np.array([0.3, 0.5, -0.1]), "separating"),     # u_N > 0: unchanged
+                     (np.array([0.01, -2.0, 0.0]), "sticking"),     # |u_T| <= -mu u_N: u -> 0
+                     (np.array([3.0, -1.0, 0.0]), "sliding")):      # radial shrink of u_T
+        p = np.zeros((6, 3))
+        p[0] = pv
+        u = pv + K
+        un = u @ nrm
+        ut = u - un * nrm
+        if un >= 0:
+            unew = u
+        elif np.linalg.norm(ut) <= -mu * un:
+            unew = np.zeros(3)
+        else:
+            unew = (1 + mu * un / np.linalg.norm(ut)) * ut
+        du = unew - u
+        pp, rc = o.project_contacts(p)
+        np.testing.assert_allclose(rc[0], du, atol=1e-12, err_msg=case)
+        np.testing.assert_allclose(pp[0], pv + G * du, atol=1e-12, err_msg=case)
+        np.testing.assert_allclose(pp[1:3], 0.0, atol=0)
