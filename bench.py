@@ -95,6 +95,15 @@ def main() -> int:
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s; ADMM iters mean {np.mean(iters):.1f}, "
           f"converged {conv}/{args.steps}, contacts last {stats[-1]['n_contacts']}", file=sys.stderr, flush=True)
 
+    # one extra (untimed) step with phase events, for the breakdown on stderr
+    backend.set_profiling(2)
+    st = backend.step()
+    ph = backend.phase_times()
+    backend.set_profiling(0)
+    print("[bench] phase ms of one step (detection, local, contact, rhs, global, other): "
+          + ", ".join(f"{v:.2f}" for v in ph) + f"; iterations {st['iterations']}, PCG {st['global_iters']}",
+          file=sys.stderr, flush=True)
+
     # roofline of the PCG SpMV (the kernel launched most and moving the most bytes)
     sizes = backend.sizes()
     n = sizes["n_dyn"]

@@ -1,0 +1,245 @@
+// Smoothed-aggregation AMG setup (host).  See pa_amg.hpp for the method.
+#include "pa_amg.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace pa {
+
+HostCsr csr_transpose(const HostCsr& A) {
+    HostCsr T;
+    T.rows = A.cols;
+    T.cols = A.rows;
+    T.ptr.assign(A.cols + 1, 0);
+    for (int32_t c : A.col) T.ptr[c + 1]++;
+    for (int i = 0; i < A.cols; i++) T.ptr[i + 1] += T.ptr[i];
+    T.col.resize(A.col.size());
+    T.val.resize(A.val.size());
+    std::vector<int32_t> fill(T.ptr.begin(), T.ptr.end() - 1);
+    for (int r = 0; r < A.rows; r++)
+        for (int e = A.ptr[r]; e < A.ptr[r + 1]; e++) {
+            int p = fill[A.col[e]]++;
+            T.col[p] = r;
+            T.val[p] = A.val[e];
+        }
+    return T; // rows of T come out with ascending columns
+}
+
+// Gustavson row-by-row product with a dense accumulator; columns sorted per row.
+HostCsr csr_multiply(const HostCsr& A, const HostCsr& B) {
+    if (A.cols != B.rows) throw std::runtime_error("csr_multiply: shape mismatch");
+    HostCsr C;
+    C.rows = A.rows;
+    C.cols = B.cols;
+    C.ptr.assign(A.rows + 1, 0);
+    std::vector<double> acc(B.cols, 0.0);
+    std::vector<int32_t> mark(B.cols, -1), cols;
+    for (int i = 0; i < A.rows; i++) {
+        cols.clear();
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
+            const int k = A.col[e];
+            const double a = A.val[e];
+            for (int f = B.ptr[k]; f < B.ptr[k + 1]; f++) {
+                const int j = B.col[f];
+                if (mark[j] != i) {
+                    mark[j] = i;
+                    acc[j] = 0.0;
+                    cols.push_back(j);
+                }
+                acc[j] += a * B.val[f];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (int j : cols) {
+            C.col.push_back(j);
+            C.val.push_back(acc[j]);
+        }
+        C.ptr[i + 1] = (int32_t)C.col.size();
+    }
+    return C;
+}
+
+double spectral_radius_dinv_a(const HostCsr& A, const std::vector<double>& dinv, int iters) {
+    // power iteration on D^-1 A (similar to the SPD D^-1/2 A D^-1/2)
+    std::vector<double> v(A.rows), w(A.rows);
+    for (int i = 0; i < A.rows; i++) v[i] = 1.0 + 0.37 * std::sin(1.3 * i); // deterministic, not an eigenvector
+    double lam = 0;
+    for (int it = 0; it < iters; it++) {
+        double nv = 0;
+        for (double q : v) nv += q * q;
+        nv = std::sqrt(nv);
+        for (double& q : v) q /= nv;
+        for (int i = 0; i < A.rows; i++) {
+            double s = 0;
+            for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) s += A.val[e] * v[A.col[e]];
+            w[i] = dinv[i] * s;
+        }
+        double num = 0;
+        for (int i = 0; i < A.rows; i++) num += w[i] * v[i] / dinv[i];
+        double den = 0;
+        for (int i = 0; i < A.rows; i++) den += v[i] * v[i] / dinv[i];
+        lam = num / den; // Rayleigh quotient in the D-inner product
+        v.swap(w);
+    }
+    return lam * 1.05; // margin for the unconverged estimate
+}
+
+namespace {
+
+// Greedy aggregation over the strength graph (three passes: seeds with fully
+// free neighbourhoods, attach to a neighbouring aggregate, leftovers).
+std::vector<int32_t> aggregate(const HostCsr& A, double theta, int* n_agg) {
+    const int n = A.rows;
+    std::vector<double> diag(n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++)
+            if (A.col[e] == i) diag[i] = A.val[e];
+    std::vector<int32_t> sptr(n + 1, 0), scol;
+    for (int i = 0; i < n; i++) {
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) {
+            const int j = A.col[e];
+            if (j != i && std::fabs(A.val[e]) >= theta * std::sqrt(std::fabs(diag[i] * diag[j]))) scol.push_back(j);
+        }
+        sptr[i + 1] = (int32_t)scol.size();
+    }
+    std::vector<int32_t> agg(n, -1);
+    int na = 0;
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        bool free_nbhd = true;
+        for (int e = sptr[i]; e < sptr[i + 1]; e++)
+            if (agg[scol[e]] >= 0) {
+                free_nbhd = false;
+                break;
+            }
+        if (!free_nbhd) continue;
+        agg[i] = na;
+        for (int e = sptr[i]; e < sptr[i + 1]; e++) agg[scol[e]] = na;
+        na++;
+    }
+    std::vector<int32_t> agg1 = agg;
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        double best = -1;
+        for (int e = sptr[i]; e < sptr[i + 1]; e++) {
+            const int j = scol[e];
+            if (agg1[j] >= 0) {
+                double s = 0;
+                for (int f = A.ptr[i]; f < A.ptr[i + 1]; f++)
+                    if (A.col[f] == j) s = std::fabs(A.val[f]);
+                if (s > best) {
+                    best = s;
+                    agg[i] = agg1[j];
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n; i++) {
+        if (agg[i] >= 0) continue;
+        agg[i] = na;
+        for (int e = sptr[i]; e < sptr[i + 1]; e++)
+            if (agg[scol[e]] < 0) agg[scol[e]] = na;
+        na++;
+    }
+    *n_agg = na;
+    return agg;
+}
+
+std::vector<double> dense_inverse_spd(const HostCsr& A) {
+    const int n = A.rows;
+    std::vector<double> M((size_t)n * n, 0.0), L((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) M[(size_t)i * n + A.col[e]] = A.val[e];
+    for (int j = 0; j < n; j++) {
+        double d = M[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        if (!(d > 0)) throw std::runtime_error("coarse operator not positive definite");
+        const double ljj = std::sqrt(d);
+        L[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; i++) {
+            double s = M[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+            L[(size_t)i * n + j] = s / ljj;
+        }
+    }
+    std::vector<double> y(n);
+    for (int c = 0; c < n; c++) { // solve L L^T x = e_c
+        for (int i = 0; i < n; i++) {
+            double s = (i == c) ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) s -= L[(size_t)i * n + k] * y[k];
+            y[i] = s / L[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[i];
+            for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
+            inv[(size_t)i * n + c] = s / L[(size_t)i * n + i];
+        }
+    }
+    return inv;
+}
+
+} // namespace
+
+AmgHierarchy amg_build(const HostCsr& A0, double theta, int coarse_max, int max_levels) {
+    AmgHierarchy H;
+    HostCsr A = A0;
+    while (true) {
+        AmgLevel L;
+        L.A = A;
+        L.dinv.assign(A.rows, 0.0);
+        for (int i = 0; i < A.rows; i++)
+            for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++)
+                if (A.col[e] == i) L.dinv[i] = 1.0 / A.val[e];
+        const double rho = spectral_radius_dinv_a(A, L.dinv);
+        L.omega = 4.0 / (3.0 * rho);
+        const bool last = A.rows <= coarse_max || (int)H.levels.size() + 1 >= max_levels;
+        if (!last) {
+            // Dense coarse operators make the strength graph dense and the
+            // aggregates tiny; raise the threshold until the level coarsens
+            // by at least 2x, else stop here.
+            int na = 0;
+            std::vector<int32_t> agg;
+            double th = theta;
+            for (int attempt = 0; attempt < 4; attempt++) {
+                agg = aggregate(A, th, &na);
+                if (na > 0 && na <= A.rows / 2) break;
+                th = std::max(2.0 * th, 0.25);
+            }
+            if (na == 0 || na > A.rows / 2) {
+                H.levels.push_back(std::move(L));
+                break;
+            }
+            HostCsr Pt;
+            Pt.rows = A.rows;
+            Pt.cols = na;
+            Pt.ptr.resize(A.rows + 1);
+            for (int i = 0; i < A.rows; i++) {
+                Pt.ptr[i + 1] = i + 1;
+                Pt.col.push_back(agg[i]);
+                Pt.val.push_back(1.0);
+            }
+            // S = I - omega D^-1 A, P = S P_tent
+            HostCsr S = A;
+            for (int i = 0; i < A.rows; i++)
+                for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++)
+                    S.val[e] = (A.col[e] == i ? 1.0 : 0.0) - L.omega * L.dinv[i] * A.val[e];
+            L.P = csr_multiply(S, Pt);
+            L.R = csr_transpose(L.P);
+            HostCsr AP = csr_multiply(A, L.P);
+            HostCsr Ac = csr_multiply(L.R, AP);
+            H.levels.push_back(std::move(L));
+            A = std::move(Ac);
+            continue;
+        }
+        H.levels.push_back(std::move(L));
+        break;
+    }
+    const HostCsr& Ac = H.levels.back().A;
+    H.coarse_n = Ac.rows;
+    if (H.coarse_n > 2048) throw std::runtime_error("AMG coarsening stalled above 2048 unknowns");
+    H.coarse_inv = dense_inverse_spd(Ac);
+    return H;
+}
+
+} // namespace pa

@@ -1,0 +1,46 @@
+// Smoothed-aggregation algebraic multigrid hierarchy for the constant system
+// matrix A = M + h^2 D^T We^2 D + h^2 Wc + Damp (built once per scene on the
+// host; the V-cycle runs on the device as the PCG preconditioner).
+//
+// Method: P. Vanek, J. Mandel, M. Brezina, "Algebraic multigrid by smoothed
+// aggregation for second and fourth order elliptic problems", Computing 56
+// (1996): strength-of-connection graph, greedy aggregation, piecewise-constant
+// tentative prolongator (the constant vector spans the near-null space: every
+// elastic row annihilates translations), Jacobi-smoothed prolongator
+// P = (I - omega D^-1 A) P_tent with omega = 4/(3 rho(D^-1 A)), Galerkin
+// coarse operator A_c = P^T A P.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace pa {
+
+struct HostCsr {
+    int rows = 0, cols = 0;
+    std::vector<int32_t> ptr{0}, col;
+    std::vector<double> val;
+    int64_t nnz() const { return (int64_t)col.size(); }
+};
+
+struct AmgLevel {
+    HostCsr A;           // operator on this level (full, diagonal included)
+    HostCsr P, R;        // prolongator to this level from the next coarser one, and R = P^T
+    std::vector<double> dinv;
+    double omega = 0.5;  // Jacobi smoothing weight 4 / (3 rho(D^-1 A))
+};
+
+struct AmgHierarchy {
+    std::vector<AmgLevel> levels; // levels[0] = fine; the last level is solved directly
+    std::vector<double> coarse_inv; // dense inverse of the coarsest operator (row-major)
+    int coarse_n = 0;
+};
+
+// theta: strength threshold |a_ij| >= theta sqrt(a_ii a_jj); coarse_max: stop coarsening below this size.
+AmgHierarchy amg_build(const HostCsr& A, double theta = 0.08, int coarse_max = 256, int max_levels = 10);
+
+HostCsr csr_multiply(const HostCsr& A, const HostCsr& B);
+HostCsr csr_transpose(const HostCsr& A);
+double spectral_radius_dinv_a(const HostCsr& A, const std::vector<double>& dinv, int iters = 30);
+
+} // namespace pa

@@ -11,6 +11,7 @@
 #include "pa_common.hpp"
 #include "pa_detector.hpp"
 #include "pa_prims.hpp"
+#include "pa_amg.hpp"
 
 namespace pa {
 
@@ -69,7 +70,23 @@ struct Backend {
     DBuf<float> cg_r, cg_z, cg_p, cg_q, cg_d;
     DBuf<double> cg_part;  // per block partial sums
     DBuf<double> cg_state; // two slots of (rz[3], done[3]); [16..18] reference norm^2, [19] iterations, [20..22] |b|^2
-    int cg_blocks = 0;
+    int cg_blocks = 0;     // grid of every kernel that writes cg_part (grid-stride over rows)
+    int a_lpr = 2;         // lanes per row for SpMVs with A
+    int last_pcg_iters = 0; // iterations of the previous solve's first pass (sizes the first launch chunk)
+    // AMG preconditioner (device copy of the hierarchy built at init)
+    struct DevLevel {
+        int n = 0, nnext = 0;
+        float omega = 0.f;
+        int lpr_a = 2, lpr_p = 2, lpr_r = 2; // lanes per row for A, P and R
+        DBuf<int32_t> aptr, acol, pptr, pcol, rptr, rcol;
+        DBuf<float> aval, dinv, pval, rval;
+        DBuf<float> b, x, x2, r; // 3 n each
+    };
+    std::vector<std::unique_ptr<DevLevel>> amg;
+    DBuf<float> amg_cinv;
+    int amg_nc = 0;
+    bool use_amg = false;
+    std::vector<int> amg_sizes;
     // contacts
     Detector* det = nullptr;
     std::unique_ptr<Detector> own_det;
@@ -115,6 +132,7 @@ struct Backend {
     void contact_sweep_once();
     void rhs(bool want_residual, bool first_iter);
     int global_solve(int max_iter, double tol);
+    void precondition(); // cg_z = M^-1 cg_r, partial r.z into cg_part[*, 6..8]
     void dual_update(bool want_residual);
     void residual_read(double* rp, double* rd);
 

@@ -15,8 +15,9 @@ struct Tree {
     DBuf<float4> node_box;   // 2 per node
     DBuf<uint32_t> keys, ids; // sorted Morton codes and primitive ids
     DBuf<int32_t> child;     // 2 per internal node
-    DBuf<int32_t> parent;    // per node
-    DBuf<uint32_t> flags;    // per internal node (bottom-up visit counters)
+    DBuf<int2> range;        // per internal node: first and last sorted leaf it covers
+    DBuf<float4> chunk64;    // 2 per 64-leaf chunk: union of its leaf boxes
+    DBuf<float4> chunk4k;    // 2 per 4096-leaf chunk
     DBuf<float4> scene;      // 2: scene box
     DBuf<float4> partial;    // 2 per reduction block
 };

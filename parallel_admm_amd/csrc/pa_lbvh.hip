@@ -125,7 +125,7 @@ __device__ inline int kdelta(const uint32_t* __restrict__ keys, const uint32_t* 
 
 // Karras construction of internal node i over sorted unique 64-bit keys.
 __global__ void k_karras(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids, int n, int32_t* __restrict__ child,
-                         int32_t* __restrict__ parent) {
+                         int2* __restrict__ range) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int d = (kdelta(keys, ids, n, i, i + 1) - kdelta(keys, ids, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -148,38 +148,63 @@ __global__ void k_karras(const uint32_t* __restrict__ keys, const uint32_t* __re
     const int right = (hi == g + 1) ? (n - 1) + g + 1 : g + 1;
     child[2 * i] = left;
     child[2 * i + 1] = right;
-    parent[left] = i;
-    parent[right] = i;
-    if (i == 0) parent[0] = -1;
+    range[i] = make_int2(lo, hi);
 }
 
-// Leaf boxes, then climb: the second thread to reach a node writes its box.
-__global__ void k_bottom_up(const float4* __restrict__ prim_box, const uint32_t* __restrict__ ids, int n,
-                            const int32_t* __restrict__ child, const int32_t* __restrict__ parent, uint32_t* __restrict__ flags,
-                            float4* __restrict__ node_box) {
+// Node boxes without cross-block synchronisation: an internal node covers
+// the contiguous sorted-leaf range [lo, hi] (Karras), so its box is the union
+// of that range, taken from the leaf boxes at the ends and from precomputed
+// unions of aligned 64- and 4096-leaf chunks in the middle.  (A bottom-up
+// climb with atomic visit flags needs device-scope fences, which write back
+// and invalidate the per-XCD L2 on this part.)
+__global__ void k_leaf_boxes(const float4* __restrict__ prim_box, const uint32_t* __restrict__ ids, int n,
+                             float4* __restrict__ node_box) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    int node = (n - 1) + k;
-    uint32_t pid = ids[k];
-    float4 lo = prim_box[2 * pid], hi = prim_box[2 * pid + 1];
-    node_box[2 * node] = lo;
-    node_box[2 * node + 1] = hi;
-    if (n == 1) return;
-    int p = parent[node];
-    while (p >= 0) {
-        __threadfence();
-        uint32_t old = atomicAdd(&flags[p], 1u);
-        __threadfence();
-        if (old == 0) return;
-        int a = child[2 * p], b = child[2 * p + 1];
-        float4 alo = node_box[2 * a], ahi = node_box[2 * a + 1];
-        float4 blo = node_box[2 * b], bhi = node_box[2 * b + 1];
-        lo = fmin4(alo, blo);
-        hi = fmax4(ahi, bhi);
-        node_box[2 * p] = lo;
-        node_box[2 * p + 1] = hi;
-        p = parent[p];
+    const int node = (n - 1) + k;
+    const uint32_t pid = ids[k];
+    node_box[2 * node] = prim_box[2 * pid];
+    node_box[2 * node + 1] = prim_box[2 * pid + 1];
+}
+
+// out[c] = union of in[c*64 .. c*64+63] (clamped to n), one wave per chunk.
+__global__ void __launch_bounds__(BLK) k_chunk_union(const float4* __restrict__ in, int n, float4* __restrict__ out, int nchunks) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wave >= nchunks) return;
+    const int e = wave * 64 + lane;
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    if (e < n) {
+        lo = in[2 * e];
+        hi = in[2 * e + 1];
     }
+    lo = wave_min4(lo);
+    hi = wave_max4(hi);
+    if (lane == 0) {
+        out[2 * wave] = lo;
+        out[2 * wave + 1] = hi;
+    }
+}
+
+__global__ void k_node_boxes(int n, const int2* __restrict__ range, const float4* __restrict__ leaf_box,
+                             const float4* __restrict__ c64, const float4* __restrict__ c4k, float4* __restrict__ node_box) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int a = range[i].x, b = range[i].y + 1; // [a, b)
+    float4 lo = make_float4(INFINITY, INFINITY, INFINITY, 0.f), hi = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+    auto take = [&](const float4* arr, int k) {
+        lo = fmin4(lo, arr[2 * k]);
+        hi = fmax4(hi, arr[2 * k + 1]);
+    };
+    // leaves up to the first 64 boundary
+    while (a < b && (a & 63)) take(leaf_box, a++);
+    while (a < b && (b & 63)) take(leaf_box, --b);
+    // 64-chunks up to the first 4096 boundary
+    int ca = a >> 6, cb = b >> 6; // chunk range [ca, cb)
+    while (ca < cb && (ca & 63)) take(c64, ca++);
+    while (ca < cb && (cb & 63)) take(c64, --cb);
+    for (int k = ca >> 6; k < (cb >> 6); k++) take(c4k, k);
+    node_box[2 * i] = lo;
+    node_box[2 * i + 1] = hi;
 }
 
 __device__ inline bool box_far(float4 nlo, float4 nhi, float4 qlo, float4 qhi, float gap) {
@@ -393,8 +418,9 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
         T.keys.alloc(n);
         T.ids.alloc(n);
         T.child.alloc(2 * (size_t)std::max(n - 1, 1));
-        T.parent.alloc(2 * (size_t)n - 1);
-        T.flags.alloc(std::max(n - 1, 1));
+        T.range.alloc(std::max(n - 1, 1));
+        T.chunk64.alloc(2 * (size_t)((n + 63) / 64));
+        T.chunk4k.alloc(2 * (size_t)((n + 4095) / 4096));
         T.scene.alloc(2);
         T.partial.alloc(2 * 256);
     };
@@ -435,14 +461,21 @@ void Detector::build_tree(Tree& T, const float* x, bool faces) {
     hipLaunchKernelGGL(k_morton, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, n, T.scene.p, T.keys.p, T.ids.p);
     PA_LAUNCH_CHECK();
     radix_sort_pairs(T.keys.p, T.ids.p, n, 30, scratch, stream);
-    if (n > 1) {
-        hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, T.keys.p, T.ids.p, n, T.child.p, T.parent.p);
-        PA_LAUNCH_CHECK();
-        fill<uint32_t>(T.flags.p, 0u, n - 1, stream);
-    }
-    hipLaunchKernelGGL(k_bottom_up, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, T.ids.p, n, T.child.p, T.parent.p,
-                       T.flags.p, T.node_box.p);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, T.ids.p, n, T.node_box.p);
     PA_LAUNCH_CHECK();
+    if (n > 1) {
+        hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, T.keys.p, T.ids.p, n, T.child.p, T.range.p);
+        PA_LAUNCH_CHECK();
+        const float4* leaves = T.node_box.p + 2 * (size_t)(n - 1);
+        const int n64 = (n + 63) / 64, n4k = (n + 4095) / 4096;
+        hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n64 * 64, BLK)), dim3(BLK), 0, stream, leaves, n, T.chunk64.p, n64);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n4k * 64, BLK)), dim3(BLK), 0, stream, T.chunk64.p, n64, T.chunk4k.p, n4k);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.range.p, leaves, T.chunk64.p, T.chunk4k.p,
+                           T.node_box.p);
+        PA_LAUNCH_CHECK();
+    }
 }
 
 int Detector::detect(const float* x) {

@@ -22,11 +22,34 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 
 namespace pa {
 namespace {
 
 constexpr int BLK = 256;
+// Upper bound on the grid of kernels that emit per-block partial sums: every
+// consumer block re-reduces all partials, so their number stays modest.
+constexpr int CG_MAX_BLOCKS = 1024;
+
+// Lanes per CSR row: about three entries per lane, so each lane's dependent
+// gather chain is short and the wave has many rows in flight.
+int lanes_per_row(int64_t nnz, int rows) {
+    const double avg = rows > 0 ? (double)nnz / rows : 0.0;
+    int l = 2;
+    while (l < 32 && l * 3 < avg) l *= 2;
+    return l;
+}
+
+template <class F> void with_lpr(int lpr, F&& f) {
+    switch (lpr) {
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: f(std::integral_constant<int, 32>{}); break;
+    }
+}
 
 // ---------------------------------------------------------------------------
 // Block reduction of K doubles (fixed order -> identical result in every block).
@@ -560,69 +583,112 @@ __global__ void __launch_bounds__(BLK) k_resid_final(const double* __restrict__ 
 // ping-pong by iteration parity): [s*8 + 0..2] rz, [s*8 + 3..5] done flags;
 // [16..18] bb; [19] iterations.
 // ---------------------------------------------------------------------------
-__device__ inline float3 a_row(int i, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col, const float* __restrict__ val,
-                               const float* __restrict__ diag, const float* __restrict__ v) {
-    float3 s = diag[i] * ld3(v, i);
-    for (int e = ptr[i]; e < ptr[i + 1]; e++) s = s + val[e] * ld3(v, col[e]);
+// Row groups: LPR consecutive lanes share one CSR row, each taking every
+// LPR-th entry, and combine with shuffles (width LPR).  Many short dependent
+// gather chains in flight instead of one long chain per lane.  Kernels that
+// emit dot-product partials run on a fixed grid (grid-stride over rows) so the
+// number of partials stays small for the redundant per-block reductions.
+template <int LPR> __device__ inline float3 group_sum(float3 s) {
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1) {
+        s.x += __shfl_xor(s.x, o, LPR);
+        s.y += __shfl_xor(s.y, o, LPR);
+        s.z += __shfl_xor(s.z, o, LPR);
+    }
     return s;
 }
 
+// sum_e val[e] * in[col[e]] over row r (lanes of the group split the entries)
+template <int LPR> __device__ inline float3 row_dot(int r, int rows, int sub, const int32_t* __restrict__ ptr,
+                                                    const int32_t* __restrict__ col, const float* __restrict__ val,
+                                                    const float* __restrict__ in) {
+    float3 s = f3(0.f, 0.f, 0.f);
+    if (r < rows)
+        for (int e = ptr[r] + sub; e < ptr[r + 1]; e += LPR) s = s + val[e] * ld3(in, col[e]);
+    return group_sum<LPR>(s);
+}
+
 // Start of one refinement pass: r = b - A x with A = hi + lo and FP64
-// accumulation (the exact residual of the FP32 iterate), then the FP32 PCG
-// for the correction d starts from d = 0: r32 = r, z = M^-1 r32, p = z.
-// Partials: r32.z (3), |r|^2 (3), |b|^2 (3).
+// accumulation (the exact residual of the FP32 iterate); the FP32 PCG for the
+// correction d starts from d = 0 with r32 = r.  Partials: |b|^2 at 0..2,
+// |r|^2 at 3..5.
+template <int LPR>
 __global__ void __launch_bounds__(BLK) k_cg_resid64(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                     const float* __restrict__ val, const float* __restrict__ val_lo,
                                                     const float* __restrict__ diag, const float* __restrict__ diag_lo,
-                                                    const float* __restrict__ dinv, const double* __restrict__ bvec,
-                                                    const float* __restrict__ x, float* __restrict__ r, float* __restrict__ zc,
-                                                    float* __restrict__ pd, float* __restrict__ d, double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < n) {
-        const double di = (double)diag[i] + (double)diag_lo[i];
-        double ax[3] = {di * x[3 * i], di * x[3 * i + 1], di * x[3 * i + 2]};
-        for (int e = ptr[i]; e < ptr[i + 1]; e++) {
-            const double a = (double)val[e] + (double)val_lo[e];
-            const int c = col[e];
-            ax[0] += a * x[3 * c];
-            ax[1] += a * x[3 * c + 1];
-            ax[2] += a * x[3 * c + 2];
-        }
-        const float dv = dinv[i];
+                                                    const double* __restrict__ bvec, const float* __restrict__ x, float* __restrict__ r,
+                                                    float* __restrict__ d, double* __restrict__ part) {
+    const int sub = threadIdx.x % LPR;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+        const int i = r0 + threadIdx.x / LPR;
+        double ax[3] = {0, 0, 0};
+        if (i < n)
+            for (int e = ptr[i] + sub; e < ptr[i + 1]; e += LPR) {
+                const double a = (double)val[e] + (double)val_lo[e];
+                const int c = col[e];
+                ax[0] += a * x[3 * c];
+                ax[1] += a * x[3 * c + 1];
+                ax[2] += a * x[3 * c + 2];
+            }
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const double bk = bvec[3 * i + k];
-            const double rk = bk - ax[k];
-            const float r32 = (float)rk;
-            const float z32 = dv * r32;
-            r[3 * i + k] = r32;
-            zc[3 * i + k] = z32;
-            pd[3 * i + k] = z32;
-            d[3 * i + k] = 0.f;
-            acc[k] = (double)r32 * z32;
-            acc[3 + k] = rk * rk;
-            acc[6 + k] = bk * bk;
+        for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+            for (int k = 0; k < 3; k++) ax[k] += __shfl_xor(ax[k], o, LPR);
+        if (i < n && sub == 0) {
+            const double di = (double)diag[i] + (double)diag_lo[i];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const double bk = bvec[3 * i + k];
+                const double rk = bk - (ax[k] + di * x[3 * i + k]);
+                r[3 * i + k] = (float)rk;
+                d[3 * i + k] = 0.f;
+                acc[k] += bk * bk;
+                acc[3 + k] += rk * rk;
+            }
         }
     }
-    block_sum_write<9>(acc, part, 9, 0);
+    block_sum_write<6>(acc, part, 9, 0);
 }
 
-// Refinement-pass state: inner stop |r_k| <= tol_in |r_0|; the pass is
-// skipped when the FP64 residual already meets |r_0| <= tol_out |b|.
-__global__ void __launch_bounds__(BLK) k_cg_init_state(const double* __restrict__ part, int nparts, double tol_in2, double tol_out2,
+// Jacobi preconditioner z = D^-1 r; partial r.z at 6..8.
+__global__ void __launch_bounds__(BLK) k_jacobi_prec(int n, const float* __restrict__ dinv, const float* __restrict__ r,
+                                                     float* __restrict__ z, double* __restrict__ part) {
+    double acc[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float di = dinv[i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float rk = r[3 * i + k], zk = di * rk;
+            z[3 * i + k] = zk;
+            acc[k] += (double)rk * zk;
+        }
+    }
+    block_sum_write<3>(acc, part, 9, 6);
+}
+
+// Pass state from the partials: rz, the stop threshold tol^2 |b|^2 (|r_0|^2
+// when b = 0) and |b|^2; a column is done at once when |r_0|^2 is already at or
+// below the threshold.
+__global__ void __launch_bounds__(BLK) k_cg_init_state(const double* __restrict__ part, int nparts, double tol2,
                                                        double* __restrict__ st) {
     double s[9];
     block_reduce_parts<9>(part, nparts, 9, 0, s);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 3; k++) {
-            st[k] = s[k];
-            st[16 + k] = tol_in2 * s[3 + k];
-            st[20 + k] = s[6 + k];
-            st[3 + k] = (s[3 + k] <= tol_out2 * s[6 + k] || s[3 + k] == 0.0) ? 1.0 : 0.0;
+            const double thr = tol2 * (s[k] > 0.0 ? s[k] : s[3 + k]);
+            st[k] = s[6 + k];
+            st[16 + k] = thr;
+            st[20 + k] = s[k];
+            st[3 + k] = (s[3 + k] <= thr || s[3 + k] == 0.0) ? 1.0 : 0.0;
         }
         st[19] = 0;
     }
+}
+
+__global__ void k_copy(int64_t m, const float* __restrict__ a, float* __restrict__ b) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) b[i] = a[i];
 }
 
 __global__ void k_axpy_d(int n, const float* __restrict__ d, float* __restrict__ x) {
@@ -631,81 +697,179 @@ __global__ void k_axpy_d(int n, const float* __restrict__ d, float* __restrict__
     x[i] += d[i];
 }
 
-// q = A pd; partial pq
+// q = A pd (off-diagonal CSR + diagonal); partial p.q at 0..2
+template <int LPR>
 __global__ void __launch_bounds__(BLK) k_cg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                  const float* __restrict__ val, const float* __restrict__ diag,
                                                  const float* __restrict__ pd, float* __restrict__ q, double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int sub = threadIdx.x % LPR;
     double acc[3] = {0, 0, 0};
-    if (i < n) {
-        float3 qi = a_row(i, ptr, col, val, diag, pd);
-        st3(q, i, qi);
-        float3 pi = ld3(pd, i);
-        acc[0] = (double)pi.x * qi.x;
-        acc[1] = (double)pi.y * qi.y;
-        acc[2] = (double)pi.z * qi.z;
+    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+        const int i = r0 + threadIdx.x / LPR;
+        float3 s = row_dot<LPR>(i, n, sub, ptr, col, val, pd);
+        if (i < n && sub == 0) {
+            const float3 pi = ld3(pd, i);
+            s = s + diag[i] * pi;
+            st3(q, i, s);
+            acc[0] += (double)pi.x * s.x;
+            acc[1] += (double)pi.y * s.y;
+            acc[2] += (double)pi.z * s.z;
+        }
     }
     block_sum_write<3>(acc, part, 9, 0);
 }
 
-// alpha = rz / pq; x += alpha pd; r -= alpha q; zc = dinv r; partial rz_new, rr
+// alpha = rz / pq; d += alpha pd; r -= alpha q; partial |r|^2 at 3..5
 __global__ void __launch_bounds__(BLK) k_cg_update(int n, int slot, int nparts, const double* __restrict__ st_in,
-                                                   const float* __restrict__ dinv, const float* __restrict__ pd,
-                                                   const float* __restrict__ q, float* __restrict__ x, float* __restrict__ r,
-                                                   float* __restrict__ zc, double* __restrict__ part) {
+                                                   const float* __restrict__ pd, const float* __restrict__ q, float* __restrict__ x,
+                                                   float* __restrict__ r, double* __restrict__ part) {
     double pq[3];
     block_reduce_parts<3>(part, nparts, 9, 0, pq);
     const double* s = st_in + slot * 8;
     float al[3];
     for (int k = 0; k < 3; k++) al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    if (i < n) {
+    double acc[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float3 pi = ld3(pd, i), qi = ld3(q, i);
         float3 xi = ld3(x, i), ri = ld3(r, i);
         xi = f3(xi.x + al[0] * pi.x, xi.y + al[1] * pi.y, xi.z + al[2] * pi.z);
         ri = f3(ri.x - al[0] * qi.x, ri.y - al[1] * qi.y, ri.z - al[2] * qi.z);
-        float3 zi = dinv[i] * ri;
         st3(x, i, xi);
         st3(r, i, ri);
-        st3(zc, i, zi);
-        acc[0] = (double)ri.x * zi.x;
-        acc[1] = (double)ri.y * zi.y;
-        acc[2] = (double)ri.z * zi.z;
-        acc[3] = (double)ri.x * ri.x;
-        acc[4] = (double)ri.y * ri.y;
-        acc[5] = (double)ri.z * ri.z;
+        acc[0] += (double)ri.x * ri.x;
+        acc[1] += (double)ri.y * ri.y;
+        acc[2] += (double)ri.z * ri.z;
     }
-    __syncthreads();
-    block_sum_write<6>(acc, part, 9, 3);
+    block_sum_write<3>(acc, part, 9, 3);
 }
 
-// beta = rz_new / rz_old; pd = zc + beta pd; block 0 advances the state slot.
+// beta = rz_new / rz_old (rz at 6..8), done when |r|^2 (3..5) <= threshold;
+// pd = zc + beta pd; block 0 advances the state slot.
 __global__ void __launch_bounds__(BLK) k_cg_dir(int n, int slot, int nparts, double* __restrict__ st,
                                                 const float* __restrict__ zc, float* __restrict__ pd, const double* __restrict__ part) {
     double s6[6];
-    block_reduce_parts<6>(part, nparts, 9, 3, s6);
+    block_reduce_parts<6>(part, nparts, 9, 3, s6); // s6[0..2] = |r|^2, s6[3..5] = r.z
     const double* so = st + slot * 8;
     float be[3];
     double done[3];
     bool all_done_before = true;
     for (int k = 0; k < 3; k++) {
-        be[k] = (so[k] == 0.0) ? 0.f : (float)(s6[k] / so[k]);
+        be[k] = (so[k] == 0.0) ? 0.f : (float)(s6[3 + k] / so[k]);
         all_done_before = all_done_before && so[3 + k] != 0.0;
-        done[k] = (so[3 + k] != 0.0 || s6[3 + k] <= st[16 + k]) ? 1.0 : 0.0;
+        done[k] = (so[3 + k] != 0.0 || s6[k] <= st[16 + k]) ? 1.0 : 0.0;
     }
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         float3 zi = ld3(zc, i), pi = ld3(pd, i);
         st3(pd, i, f3(zi.x + be[0] * pi.x, zi.y + be[1] * pi.y, zi.z + be[2] * pi.z));
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         double* sn = st + (slot ^ 1) * 8;
         for (int k = 0; k < 3; k++) {
-            sn[k] = s6[k];
+            sn[k] = s6[3 + k];
             sn[3 + k] = done[k];
         }
         if (!all_done_before) st[19] += 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Smoothed-aggregation AMG V-cycle (one weighted-Jacobi pre- and post-sweep per
+// level, dense inverse on the coarsest level); symmetric, so usable as the PCG
+// preconditioner.  Level matrices hold the diagonal.
+// ---------------------------------------------------------------------------
+// Pre-smoothing from a zero guess: x = w D^-1 b; r = b - A x.  The scaled
+// input w D^-1 b is formed on the fly for each gathered neighbour.
+template <int LPR>
+__global__ void __launch_bounds__(BLK) k_amg_pre(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                 const float* __restrict__ val, const float* __restrict__ dinv, float w,
+                                                 const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r) {
+    const int sub = threadIdx.x % LPR;
+    const int i = blockIdx.x * (BLK / LPR) + threadIdx.x / LPR;
+    float3 s = f3(0.f, 0.f, 0.f);
+    if (i < n)
+        for (int e = ptr[i] + sub; e < ptr[i + 1]; e += LPR) {
+            const int c = col[e];
+            s = s + (val[e] * (w * dinv[c])) * ld3(b, c);
+        }
+    s = group_sum<LPR>(s);
+    if (i < n && sub == 0) {
+        const float3 bi = ld3(b, i);
+        st3(x, i, (w * dinv[i]) * bi);
+        st3(r, i, bi - s);
+    }
+}
+
+// out = base + M in (M a CSR with `rows` rows); base may be null.
+template <int LPR>
+__global__ void __launch_bounds__(BLK) k_amg_spmv(int rows, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                  const float* __restrict__ val, const float* __restrict__ in,
+                                                  const float* __restrict__ base, float* __restrict__ out) {
+    const int sub = threadIdx.x % LPR;
+    const int i = blockIdx.x * (BLK / LPR) + threadIdx.x / LPR;
+    float3 s = row_dot<LPR>(i, rows, sub, ptr, col, val, in);
+    if (i < rows && sub == 0) st3(out, i, base ? ld3(base, i) + s : s);
+}
+
+// Post-smoothing: xo = xi + w D^-1 (b - A xi); with part != null also b.xo at
+// 6..8 (on the finest level b is the PCG residual, so this is r.z).
+template <int LPR>
+__global__ void __launch_bounds__(BLK) k_amg_post(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                  const float* __restrict__ val, const float* __restrict__ dinv, float w,
+                                                  const float* __restrict__ b, const float* __restrict__ xi, float* __restrict__ xo,
+                                                  double* __restrict__ part) {
+    const int sub = threadIdx.x % LPR;
+    double acc[3] = {0, 0, 0};
+    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+        const int i = r0 + threadIdx.x / LPR;
+        const float3 ax = row_dot<LPR>(i, n, sub, ptr, col, val, xi);
+        if (i < n && sub == 0) {
+            const float3 bi = ld3(b, i);
+            const float3 v = ld3(xi, i) + (w * dinv[i]) * (bi - ax);
+            st3(xo, i, v);
+            acc[0] += (double)bi.x * v.x;
+            acc[1] += (double)bi.y * v.y;
+            acc[2] += (double)bi.z * v.z;
+        }
+    }
+    if (part) block_sum_write<3>(acc, part, 9, 6);
+}
+
+// Partial r.z at 6..8.
+__global__ void __launch_bounds__(BLK) k_rz_part(int n, const float* __restrict__ r, const float* __restrict__ z,
+                                                 double* __restrict__ part) {
+    double acc[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = 0; k < 3; k++) acc[k] += (double)r[3 * i + k] * z[3 * i + k];
+    block_sum_write<3>(acc, part, 9, 6);
+}
+
+// Coarsest level: x = A_c^-1 b with the dense inverse (nc <= 2048), one wave
+// per row reading the row with coalesced loads; b is staged in LDS.
+__global__ void __launch_bounds__(BLK) k_amg_coarse(int nc, const float* __restrict__ inv, const float* __restrict__ b,
+                                                    float* __restrict__ x) {
+    __shared__ float sb[3 * 2048];
+    for (int j = threadIdx.x; j < 3 * nc; j += BLK) sb[j] = b[j];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
+    if (row >= nc) return;
+    const float* rp = inv + (size_t)row * nc;
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
+    for (int j = lane; j < nc; j += 64) {
+        const float a = rp[j];
+        s0 += a * sb[3 * j];
+        s1 += a * sb[3 * j + 1];
+        s2 += a * sb[3 * j + 2];
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o, 64);
+        s1 += __shfl_xor(s1, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    if (lane == 0) {
+        x[3 * row] = s0;
+        x[3 * row + 1] = s1;
+        x[3 * row + 2] = s2;
     }
 }
 
@@ -951,6 +1115,60 @@ void Backend::assemble_and_upload(const double* x0h) {
     }
     nnz_a_off = (int64_t)ocol.size();
     nnz_dm = (int64_t)dcol.size();
+    // AMG hierarchy on the full double-precision matrix
+    use_amg = false;
+    amg.clear();
+    amg_sizes.clear();
+    if (cfg.precond == 1) {
+        HostCsr Ah;
+        Ah.rows = Ah.cols = n;
+        Ah.ptr.assign(aptr.begin(), aptr.end());
+        Ah.col.assign(acol.begin(), acol.end());
+        Ah.val.assign(aval.begin(), aval.end());
+        AmgHierarchy H;
+        bool ok = true;
+        try {
+            H = amg_build(Ah);
+        } catch (const std::exception&) {
+            ok = false;
+        }
+        if (ok && H.coarse_n <= 2048) {
+            for (size_t l = 0; l < H.levels.size(); l++) {
+                const AmgLevel& L = H.levels[l];
+                auto D = std::make_unique<DevLevel>();
+                D->n = L.A.rows;
+                D->omega = (float)L.omega;
+                D->lpr_a = lanes_per_row(L.A.nnz(), L.A.rows);
+                amg_sizes.push_back(D->n);
+                std::vector<float> av(L.A.val.begin(), L.A.val.end()), dv(L.dinv.begin(), L.dinv.end());
+                D->aptr.upload(L.A.ptr, stream);
+                D->acol.upload(L.A.col, stream);
+                D->aval.upload(av, stream);
+                D->dinv.upload(dv, stream);
+                if (l + 1 < H.levels.size()) {
+                    D->nnext = L.P.cols;
+                    D->lpr_p = lanes_per_row(L.P.nnz(), L.P.rows);
+                    D->lpr_r = lanes_per_row(L.R.nnz(), L.R.rows);
+                    std::vector<float> pv(L.P.val.begin(), L.P.val.end()), rv(L.R.val.begin(), L.R.val.end());
+                    D->pptr.upload(L.P.ptr, stream);
+                    D->pcol.upload(L.P.col, stream);
+                    D->pval.upload(pv, stream);
+                    D->rptr.upload(L.R.ptr, stream);
+                    D->rcol.upload(L.R.col, stream);
+                    D->rval.upload(rv, stream);
+                }
+                D->b.alloc(3 * (size_t)D->n);
+                D->x.alloc(3 * (size_t)D->n);
+                D->x2.alloc(3 * (size_t)D->n);
+                D->r.alloc(3 * (size_t)D->n);
+                amg.push_back(std::move(D));
+            }
+            std::vector<float> ci(H.coarse_inv.begin(), H.coarse_inv.end());
+            amg_cinv.upload(ci, stream);
+            amg_nc = H.coarse_n;
+            use_amg = true;
+        }
+    }
     // D^T We^2 h^2 gather, per vertex sorted by row
     std::vector<std::array<double, 3>> dt; // (v, row, coef)
     dt.reserve(ents.size());
@@ -1053,7 +1271,8 @@ void Backend::assemble_and_upload(const double* x0h) {
     cg_p.alloc(3 * (size_t)n);
     cg_q.alloc(3 * (size_t)n);
     cg_d.alloc(3 * (size_t)n);
-    cg_blocks = grid_for(n, BLK);
+    a_lpr = lanes_per_row(nnz_a_off + n, n);
+    cg_blocks = std::max(1, std::min(grid_for((int64_t)n * a_lpr, BLK), CG_MAX_BLOCKS));
     cg_part.alloc(9 * (size_t)cg_blocks);
     cg_state.alloc(24);
     int elem_blocks = grid_for(hc.n_tri, BLK) + grid_for(hc.n_tet, BLK) + grid_for(hc.n_bend, BLK) + grid_for(hc.n_pin, BLK);
@@ -1195,28 +1414,99 @@ hipEvent_t Backend::next_event() {
 // FP32 iterate, solves A d = r with FP32 PCG to |r_k| <= global_tol |r_0|,
 // and adds d.  Passes stop early once |r| <= 1e-12 |b| (FP64).  Returns the
 // PCG iterations summed over passes.
+void Backend::precondition() {
+    const int nb = cg_blocks;
+    if (!use_amg) {
+        hipLaunchKernelGGL(k_jacobi_prec, dim3(nb), dim3(BLK), 0, stream, n, a_dinv.p, cg_r.p, cg_z.p, cg_part.p);
+        PA_LAUNCH_CHECK();
+        return;
+    }
+    const int nl = (int)amg.size();
+    // downward: level 0 takes b = cg_r
+    for (int l = 0; l + 1 < nl; l++) {
+        DevLevel& L = *amg[l];
+        DevLevel& C = *amg[l + 1];
+        const float* bl = l == 0 ? cg_r.p : L.b.p;
+        with_lpr(L.lpr_a, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_amg_pre<P>, dim3(grid_for((int64_t)L.n * P, BLK)), dim3(BLK), 0, stream, L.n, L.aptr.p, L.acol.p,
+                               L.aval.p, L.dinv.p, L.omega, bl, L.x.p, L.r.p);
+        });
+        PA_LAUNCH_CHECK();
+        with_lpr(L.lpr_r, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_amg_spmv<P>, dim3(grid_for((int64_t)L.nnext * P, BLK)), dim3(BLK), 0, stream, L.nnext, L.rptr.p,
+                               L.rcol.p, L.rval.p, (const float*)L.r.p, (const float*)nullptr, C.b.p);
+        });
+        PA_LAUNCH_CHECK();
+    }
+    DevLevel& Cl = *amg[nl - 1];
+    const float* bc = nl == 1 ? cg_r.p : Cl.b.p;
+    float* xc = nl == 1 ? cg_z.p : Cl.x2.p;
+    hipLaunchKernelGGL(k_amg_coarse, dim3(grid_for(amg_nc, BLK / 64)), dim3(BLK), 0, stream, amg_nc, amg_cinv.p, bc, xc);
+    PA_LAUNCH_CHECK();
+    if (nl == 1) { // the whole system is the coarse level: exact solve, then r.z partials
+        hipLaunchKernelGGL(k_rz_part, dim3(nb), dim3(BLK), 0, stream, n, cg_r.p, cg_z.p, cg_part.p);
+        PA_LAUNCH_CHECK();
+        return;
+    }
+    // upward: x_l += P x_{l+1}, then post-smooth into x2_l (cg_z on level 0)
+    for (int l = nl - 2; l >= 0; l--) {
+        DevLevel& L = *amg[l];
+        const float* bl = l == 0 ? cg_r.p : L.b.p;
+        with_lpr(L.lpr_p, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_amg_spmv<P>, dim3(grid_for((int64_t)L.n * P, BLK)), dim3(BLK), 0, stream, L.n, L.pptr.p, L.pcol.p,
+                               L.pval.p, (const float*)amg[l + 1]->x2.p, (const float*)L.x.p, L.x.p);
+        });
+        PA_LAUNCH_CHECK();
+        float* out = l == 0 ? cg_z.p : L.x2.p;
+        // level 0 emits r.z partials, so it runs on the partial-sum grid
+        with_lpr(L.lpr_a, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            const int grid = l == 0 ? nb : grid_for((int64_t)L.n * P, BLK);
+            hipLaunchKernelGGL(k_amg_post<P>, dim3(grid), dim3(BLK), 0, stream, L.n, L.aptr.p, L.acol.p, L.aval.p, L.dinv.p, L.omega,
+                               bl, (const float*)L.x.p, out, l == 0 ? cg_part.p : (double*)nullptr);
+        });
+        PA_LAUNCH_CHECK();
+    }
+}
+
+// Mixed-precision refinement: each pass computes the FP64 residual
+// r = b - A x of the FP32 iterate, solves A d = r with preconditioned FP32 CG
+// until |r_k| <= tol |b| per column (the recursive residual), and adds d.  A
+// later pass runs only while the FP64 residual is still above tol |b|, which
+// catches the drift between recursive and true residual.  Iterations are
+// launched in chunks between convergence read-backs; the first chunk is sized
+// by the previous solve's count (consecutive ADMM iterations need similar
+// counts).  Returns the PCG iterations summed over passes.
 int Backend::global_solve(int max_iter, double tol) {
-    const double tol_in2 = tol * tol, tol_out2 = 1e-24;
     const int nb = cg_blocks;
     const int passes = std::max(1, cfg.global_refine);
     int total_iters = 0;
     double hs[24];
     for (int pass = 0; pass < passes; pass++) {
-        hipLaunchKernelGGL(k_cg_resid64, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p, a_diag_lo.p,
-                           a_dinv.p, b.p, x.p, cg_r.p, cg_z.p, cg_p.p, cg_d.p, cg_part.p);
+        with_lpr(a_lpr, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_cg_resid64<P>, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p,
+                               a_diag_lo.p, b.p, x.p, cg_r.p, cg_d.p, cg_part.p);
+        });
         PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(BLK), 0, stream, cg_part.p, nb, tol_in2, tol_out2, cg_state.p);
+        precondition();
+        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(BLK), 0, stream, cg_part.p, nb, tol * tol, cg_state.p);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_copy, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, 3 * (int64_t)n, cg_z.p, cg_p.p);
         PA_LAUNCH_CHECK();
         if (pass > 0) {
-            // a later pass runs only if the FP64 residual still exceeds the floor
             PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
             PA_HIP(hipStreamSynchronize(stream));
             if (hs[3] != 0 && hs[4] != 0 && hs[5] != 0) break;
         }
-        const int chunk = 8;
+        int chunk = std::max(2, pass == 0 ? last_pcg_iters : 2);
         int it = 0;
         while (it < max_iter) {
-            int stop = std::min(max_iter, it + chunk);
+            const int stop = std::min(max_iter, it + chunk);
+            chunk = 2;
             for (; it < stop; it++) {
                 const int slot = it & 1;
                 hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1225,16 +1515,20 @@ int Backend::global_solve(int max_iter, double tol) {
                     e1 = next_event();
                     PA_HIP(hipEventRecord(e0, stream));
                 }
-                hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p, cg_p.p, cg_q.p,
-                                   cg_part.p);
+                with_lpr(a_lpr, [&](auto c) {
+                    constexpr int P = decltype(c)::value;
+                    hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
+                                       (const float*)cg_p.p, cg_q.p, cg_part.p);
+                });
                 PA_LAUNCH_CHECK();
                 if (profiling & 1) {
                     PA_HIP(hipEventRecord(e1, stream));
                     spmv_marks.push_back({e0, e1});
                 }
-                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, a_dinv.p, cg_p.p, cg_q.p, cg_d.p,
-                                   cg_r.p, cg_z.p, cg_part.p);
+                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, cg_p.p, cg_q.p, cg_d.p, cg_r.p,
+                                   cg_part.p);
                 PA_LAUNCH_CHECK();
+                precondition();
                 hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
                 PA_LAUNCH_CHECK();
             }
@@ -1243,6 +1537,7 @@ int Backend::global_solve(int max_iter, double tol) {
             const double* sl = hs + (it & 1) * 8;
             if (sl[3] != 0 && sl[4] != 0 && sl[5] != 0) break;
         }
+        if (pass == 0) last_pcg_iters = (int)hs[19];
         total_iters += (int)hs[19];
         hipLaunchKernelGGL(k_axpy_d, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, n, cg_d.p, x.p);
         PA_LAUNCH_CHECK();

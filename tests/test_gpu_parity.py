@@ -145,17 +145,20 @@ def test_contact_projection_matches_oracle(pa):
     assert e1 < 1e-4 and e2 < 1e-4
 
 
-def test_global_solve_matches_llt(pa):
+@pytest.mark.parametrize("precond", [0, 1], ids=["jacobi", "amg"])
+def test_global_solve_matches_llt(pa, precond):
     """Global solve (FP64 residual + FP32 PCG correction, two refinement passes)
-    against the oracle's sparse LLT: relative error <= 1e-6."""
+    against the oracle's sparse LLT: relative error <= 1e-6, with the Jacobi
+    and the smoothed-aggregation AMG preconditioners."""
     from parallel_admm_amd.configs import config_for
 
     rng = np.random.default_rng(5)
-    for kind, name in ((1, "cloth32"), (2, "tetbox")):
+    for kind, name in ((1, "cloth32"), (2, "tetbox"), (3, "cloth256")):
         cfg = config_for(name)
-        cfg.global_tol = 1e-5
+        cfg.global_tol = 1e-8
         cfg.global_refine = 2
         cfg.global_max_iter = 2000
+        cfg.precond = precond
         sc = pa.Scene.generate(kind)
         m = sc.mesh()
         b = pa.ADMMBackendHIP.from_scene(sc, cfg)
@@ -167,8 +170,10 @@ def test_global_solve_matches_llt(pa):
         xo = o.solve(rhs)
         xg, it = b.global_solve(rhs, m["x"][:n])
         err = rel_err(xg, xo)
-        print(f"{name}: PCG {it} iterations, rel err vs LLT {err:.2e}")
+        print(f"{name} precond={precond}: PCG {it} iterations, rel err vs LLT {err:.2e}")
         assert err < 1e-6
+        if precond == 1:
+            assert it <= 60
 
 
 STEP_CASES = [
