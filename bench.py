@@ -25,10 +25,12 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
 def spmv_bytes(n: int, nnz_off: int) -> int:
-    """Algorithmic bytes of one PCG SpMV launch q = A p (3 interleaved columns, FP32):
+    """Algorithmic bytes of one launch of the fused PCG direction + SpMV kernel
+    (k_cg_dir_spmv: p' = z + beta p, q = A p', 3 interleaved FP32 columns):
     off-diagonal values + column ids (8 B each), row pointers (4(n+1)), diagonal
-    (4n), p read once (12n), q written (12n), per-block partial sums (negligible)."""
-    return 8 * nnz_off + 4 * (n + 1) + 4 * n + 12 * n + 12 * n
+    (4n), z and p read once (24n), p' and q written (24n); the per-block
+    partial sums are negligible."""
+    return 8 * nnz_off + 4 * (n + 1) + 4 * n + 24 * n + 24 * n
 
 
 def cpu_baseline(workload: str, scene, cfg, seconds: float = 20.0) -> dict:
@@ -104,7 +106,7 @@ def main() -> int:
           + ", ".join(f"{v:.2f}" for v in ph) + f"; iterations {st['iterations']}, PCG {st['global_iters']}",
           file=sys.stderr, flush=True)
 
-    # roofline of the PCG SpMV (the kernel launched most and moving the most bytes)
+    # roofline of the PCG SpMV (the kernel moving the most bytes per launch in the global solve)
     sizes = backend.sizes()
     n = sizes["n_dyn"]
     nnz_off = sizes["nnz_a_off"]
@@ -144,7 +146,7 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_spmv",
+            "kernel": "k_cg_dir_spmv",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -10,7 +10,7 @@
 //
 // Pinning (which independent check fixes each part):
 //   dist_pt / dist_ee        closed forms + brute-force (s,t) grid minimum (tests/test_oracle_geometry.py)
-//   detect (contact set)     O(N^2) loop with no broad phase == broad-phase result; invariants
+//   detect (contact set)     O(N^2) loop with no culling (the definition) == uniform-grid broad phase; invariants
 //   tri/tet prox             polar-decomposition invariants, closed forms F = R*diag, F = I
 //   bend / pin prox          closed forms (z/2, pin position)
 //   sparse LLT               ||A x - b|| / ||b|| <= 1e-12 against a dense reference (numpy)
@@ -1067,33 +1067,120 @@ std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute
     for (int e = 0; e < ne; e++) eb[e] = boxof(T.surface_edges[e].data(), 2);
     const R gap = dhat * R(1.001) + R(1e-12);
     std::vector<std::vector<Contact<R>>> pt_out(nv), ee_out(ne);
+    if (brute_force) {
+        // the definition: every pair, no culling
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int qi = 0; qi < nv; qi++) {
-        int v = T.surface_vinds[qi];
-        V3<R> p = load3(x, v);
-        Box<R> pb;
-        pb.lo[0] = pb.hi[0] = p.x;
-        pb.lo[1] = pb.hi[1] = p.y;
-        pb.lo[2] = pb.hi[2] = p.z;
-        for (int f = 0; f < nf; f++) {
-            const auto& fa = T.surface_tris[f];
-            if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
-            if (T.is_static[v] && T.is_static[fa[0]]) continue;
-            if (!brute_force && boxes_far(pb, fb[f], gap)) continue;
-            PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
-            if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
+        for (int qi = 0; qi < nv; qi++) {
+            int v = T.surface_vinds[qi];
+            V3<R> p = load3(x, v);
+            for (int f = 0; f < nf; f++) {
+                const auto& fa = T.surface_tris[f];
+                if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
+                if (T.is_static[v] && T.is_static[fa[0]]) continue;
+                PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
+                if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
+            }
         }
-    }
 #pragma omp parallel for schedule(dynamic, 16)
-    for (int i = 0; i < ne; i++) {
-        const auto& e0 = T.surface_edges[i];
-        for (int j = i + 1; j < ne; j++) {
-            const auto& e1 = T.surface_edges[j];
-            if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
-            if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
-            if (!brute_force && boxes_far(eb[i], eb[j], gap)) continue;
-            EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
-            if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
+        for (int i = 0; i < ne; i++) {
+            const auto& e0 = T.surface_edges[i];
+            for (int j = i + 1; j < ne; j++) {
+                const auto& e1 = T.surface_edges[j];
+                if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
+                if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
+                EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
+                if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
+            }
+        }
+    } else {
+        // Uniform-grid broad phase: every face and edge box, grown by the gap,
+        // is entered into each cell it overlaps; a query visits the cells of its
+        // own (ungrown) box.  Two boxes within `gap` of each other share a cell,
+        // so the candidates are a superset of the box-overlap pairs and the
+        // exact predicate gives the brute-force set.  Cell size: the median
+        // surface edge length, at least 4 gap.
+        std::vector<R> elen(ne);
+        for (int e = 0; e < ne; e++) elen[e] = norm(load3(x, T.surface_edges[e][1]) - load3(x, T.surface_edges[e][0]));
+        R cell = R(4) * gap;
+        if (ne > 0) {
+            std::nth_element(elen.begin(), elen.begin() + ne / 2, elen.end());
+            cell = std::max(cell, elen[ne / 2]);
+        }
+        auto cidx = [&](R c) {
+            R q = std::floor(c / cell);
+            q = std::min(std::max(q, R(-(1 << 20))), R((1 << 20) - 1));
+            return (int64_t)q + (1 << 20);
+        };
+        auto key = [](int64_t a, int64_t b, int64_t c) { return (a << 42) | (b << 21) | c; };
+        auto build = [&](const std::vector<Box<R>>& bx, std::vector<std::pair<int64_t, int32_t>>& tab) {
+            tab.clear();
+            for (int k = 0; k < (int)bx.size(); k++) {
+                int64_t lo[3], hi[3];
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = cidx(bx[k].lo[a] - gap);
+                    hi[a] = cidx(bx[k].hi[a] + gap);
+                }
+                for (int64_t i = lo[0]; i <= hi[0]; i++)
+                    for (int64_t j = lo[1]; j <= hi[1]; j++)
+                        for (int64_t l = lo[2]; l <= hi[2]; l++) tab.push_back({key(i, j, l), k});
+            }
+            std::sort(tab.begin(), tab.end());
+        };
+        auto cell_range = [](const std::vector<std::pair<int64_t, int32_t>>& tab, int64_t k) {
+            auto a = std::lower_bound(tab.begin(), tab.end(), std::make_pair(k, (int32_t)INT32_MIN));
+            auto b = std::upper_bound(a, tab.end(), std::make_pair(k, (int32_t)INT32_MAX));
+            return std::make_pair(a, b);
+        };
+        std::vector<std::pair<int64_t, int32_t>> ftab, etab;
+        build(fb, ftab);
+        build(eb, etab);
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int qi = 0; qi < nv; qi++) {
+            int v = T.surface_vinds[qi];
+            V3<R> p = load3(x, v);
+            Box<R> pb;
+            pb.lo[0] = pb.hi[0] = p.x;
+            pb.lo[1] = pb.hi[1] = p.y;
+            pb.lo[2] = pb.hi[2] = p.z;
+            auto rg = cell_range(ftab, key(cidx(p.x), cidx(p.y), cidx(p.z)));
+            std::vector<int32_t> cand;
+            for (auto it = rg.first; it != rg.second; ++it) cand.push_back(it->second);
+            std::sort(cand.begin(), cand.end()); // faces in ascending order, as the brute-force loop
+            for (int f : cand) {
+                const auto& fa = T.surface_tris[f];
+                if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
+                if (T.is_static[v] && T.is_static[fa[0]]) continue;
+                if (boxes_far(pb, fb[f], gap)) continue;
+                PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
+                if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 64)
+        for (int i = 0; i < ne; i++) {
+            const auto& e0 = T.surface_edges[i];
+            int64_t lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                lo[a] = cidx(eb[i].lo[a]);
+                hi[a] = cidx(eb[i].hi[a]);
+            }
+            std::vector<int32_t> cand;
+            for (int64_t a = lo[0]; a <= hi[0]; a++)
+                for (int64_t b = lo[1]; b <= hi[1]; b++)
+                    for (int64_t c = lo[2]; c <= hi[2]; c++) {
+                        auto rg = cell_range(etab, key(a, b, c));
+                        for (auto it = rg.first; it != rg.second; ++it)
+                            if (it->second > i) cand.push_back(it->second);
+                    }
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (int j : cand) {
+                const auto& e1 = T.surface_edges[j];
+                if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
+                if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
+                if (boxes_far(eb[i], eb[j], gap)) continue;
+                EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
+                if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
+            }
         }
     }
     std::vector<Contact<R>> out;

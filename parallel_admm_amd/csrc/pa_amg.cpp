@@ -146,34 +146,53 @@ std::vector<int32_t> aggregate(const HostCsr& A, double theta, int* n_agg) {
     return agg;
 }
 
+// A^-1 = L^-T L^-1 from the Cholesky factor A = L L^T; every inner loop runs
+// over contiguous memory (row-major L, and W = L^-1 stored by rows).
 std::vector<double> dense_inverse_spd(const HostCsr& A) {
     const int n = A.rows;
-    std::vector<double> M((size_t)n * n, 0.0), L((size_t)n * n, 0.0), inv((size_t)n * n, 0.0);
+    std::vector<double> L((size_t)n * n, 0.0);
     for (int i = 0; i < n; i++)
-        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++) M[(size_t)i * n + A.col[e]] = A.val[e];
-    for (int j = 0; j < n; j++) {
-        double d = M[(size_t)j * n + j];
-        for (int k = 0; k < j; k++) d -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
-        if (!(d > 0)) throw std::runtime_error("coarse operator not positive definite");
-        const double ljj = std::sqrt(d);
-        L[(size_t)j * n + j] = ljj;
-        for (int i = j + 1; i < n; i++) {
-            double s = M[(size_t)i * n + j];
-            for (int k = 0; k < j; k++) s -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
-            L[(size_t)i * n + j] = s / ljj;
+        for (int e = A.ptr[i]; e < A.ptr[i + 1]; e++)
+            if (A.col[e] <= i) L[(size_t)i * n + A.col[e]] = A.val[e];
+    // row-oriented Cholesky: L[i][j] = (A[i][j] - L[i][:j] . L[j][:j]) / L[j][j]
+    for (int i = 0; i < n; i++) {
+        double* Li = &L[(size_t)i * n];
+        for (int j = 0; j <= i; j++) {
+            const double* Lj = &L[(size_t)j * n];
+            double s = Li[j];
+            for (int k = 0; k < j; k++) s -= Li[k] * Lj[k];
+            if (j < i) {
+                Li[j] = s / Lj[j];
+            } else {
+                if (!(s > 0)) throw std::runtime_error("coarse operator not positive definite");
+                Li[i] = std::sqrt(s);
+            }
         }
     }
-    std::vector<double> y(n);
-    for (int c = 0; c < n; c++) { // solve L L^T x = e_c
-        for (int i = 0; i < n; i++) {
-            double s = (i == c) ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) s -= L[(size_t)i * n + k] * y[k];
-            y[i] = s / L[(size_t)i * n + i];
+    // W = L^-1 (lower triangular), row by row: W[i][:] = (e_i - L[i][:i] W[:i][:]) / L[i][i]
+    std::vector<double> W((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++) {
+        double* Wi = &W[(size_t)i * n];
+        const double* Li = &L[(size_t)i * n];
+        Wi[i] = 1.0;
+        for (int k = 0; k < i; k++) {
+            const double lik = Li[k];
+            if (lik == 0.0) continue;
+            const double* Wk = &W[(size_t)k * n];
+            for (int c = 0; c <= k; c++) Wi[c] -= lik * Wk[c];
         }
-        for (int i = n - 1; i >= 0; i--) {
-            double s = y[i];
-            for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * inv[(size_t)k * n + c];
-            inv[(size_t)i * n + c] = s / L[(size_t)i * n + i];
+        const double inv = 1.0 / Li[i];
+        for (int c = 0; c <= i; c++) Wi[c] *= inv;
+    }
+    // inv = W^T W: inv[a][b] = sum_k W[k][a] W[k][b], accumulated row by row of W
+    std::vector<double> inv((size_t)n * n, 0.0);
+    for (int k = 0; k < n; k++) {
+        const double* Wk = &W[(size_t)k * n];
+        for (int a = 0; a <= k; a++) {
+            const double wa = Wk[a];
+            if (wa == 0.0) continue;
+            double* Ia = &inv[(size_t)a * n];
+            for (int b = 0; b <= k; b++) Ia[b] += wa * Wk[b];
         }
     }
     return inv;

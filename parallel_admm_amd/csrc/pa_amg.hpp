@@ -37,7 +37,7 @@ struct AmgHierarchy {
 };
 
 // theta: strength threshold |a_ij| >= theta sqrt(a_ii a_jj); coarse_max: stop coarsening below this size.
-AmgHierarchy amg_build(const HostCsr& A, double theta = 0.08, int coarse_max = 256, int max_levels = 10);
+AmgHierarchy amg_build(const HostCsr& A, double theta = 0.08, int coarse_max = 1536, int max_levels = 10);
 
 HostCsr csr_multiply(const HostCsr& A, const HostCsr& B);
 HostCsr csr_transpose(const HostCsr& A);

@@ -68,8 +68,9 @@ struct Backend {
     int64_t nnz_a_off = 0, nnz_dm = 0;
     // PCG
     DBuf<float> cg_r, cg_z, cg_p, cg_q, cg_d;
+    DBuf<float> cg_r2, cg_p2; // ping-pong partners of cg_r and cg_p for the fused AMG iteration
     DBuf<double> cg_part;  // per block partial sums
-    DBuf<double> cg_state; // two slots of (rz[3], done[3]); [16..18] reference norm^2, [19] iterations, [20..22] |b|^2
+    DBuf<double> cg_state; // two slots of (rz[3], done[3]); [16..18] threshold, [19] iterations, [20..22] |b|^2
     int cg_blocks = 0;     // grid of every kernel that writes cg_part (grid-stride over rows)
     int a_lpr = 2;         // lanes per row for SpMVs with A
     int last_pcg_iters = 0; // iterations of the previous solve's first pass (sizes the first launch chunk)
@@ -84,7 +85,7 @@ struct Backend {
     };
     std::vector<std::unique_ptr<DevLevel>> amg;
     DBuf<float> amg_cinv;
-    int amg_nc = 0;
+    int amg_nc = 0, amg_ncp = 0; // coarsest size and its row stride (multiple of 4)
     bool use_amg = false;
     std::vector<int> amg_sizes;
     // contacts
@@ -132,7 +133,9 @@ struct Backend {
     void contact_sweep_once();
     void rhs(bool want_residual, bool first_iter);
     int global_solve(int max_iter, double tol);
-    void precondition(); // cg_z = M^-1 cg_r, partial r.z into cg_part[*, 6..8]
+    // cg_z = M^-1 r, partial r.z into cg_part[*, 6..8]; with skip_pre0 the
+    // level-0 pre-smoothing was already done by k_cg_update_pre.
+    void precondition(const float* r, bool skip_pre0 = false);
     void dual_update(bool want_residual);
     void residual_read(double* rp, double* rd);
 

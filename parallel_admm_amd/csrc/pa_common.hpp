@@ -71,9 +71,14 @@ template <class T> struct DBuf {
         p = nullptr;
         n = 0;
     }
+    // Copies from pageable host memory and waits, so the caller may free or
+    // reuse `h` on return (uploads happen at set-up and on state transfers).
     void upload(const T* h, size_t count, hipStream_t s) {
         PA_CHECK(count <= n, PA_ERR_INVALID, "upload larger than buffer");
-        if (count) PA_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+        if (count) {
+            PA_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+            PA_HIP(hipStreamSynchronize(s));
+        }
     }
     void upload(const std::vector<T>& h, hipStream_t s) {
         reserve(h.size());

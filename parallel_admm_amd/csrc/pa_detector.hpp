@@ -16,6 +16,7 @@ struct Tree {
     DBuf<uint32_t> keys, ids; // sorted Morton codes and primitive ids
     DBuf<int32_t> child;     // 2 per internal node
     DBuf<int2> range;        // per internal node: first and last sorted leaf it covers
+    DBuf<float4> rec;        // 4 per internal node: child boxes, child ids, last leaf under each child
     DBuf<float4> chunk64;    // 2 per 64-leaf chunk: union of its leaf boxes
     DBuf<float4> chunk4k;    // 2 per 4096-leaf chunk
     DBuf<float4> scene;      // 2: scene box
@@ -41,7 +42,8 @@ struct Detector {
     DBuf<float> x_own; // positions uploaded through detect(host x)
     Tree ft, et;
     ContactBuffers c;
-    DBuf<uint32_t> qcount, qoffset; // nv + ne query counts / offsets
+    DBuf<uint32_t> qcount, qoffset; // nv + ne segment counts / offsets (PT per query vertex, EE per smaller edge id)
+    DBuf<uint32_t> qfill;           // ne: EE slots taken per segment during the write pass
     DBuf<uint32_t> total;           // [0] = contacts found (may exceed cap), [1] = stack overflow flag
     PrimScratch scratch;
     int n_contacts = 0, peak = 0;

@@ -3,10 +3,18 @@
 // Per detection: primitive boxes -> scene box -> 30-bit Morton codes ->
 // stable LDS radix sort -> Karras radix tree (T. Karras, "Maximizing
 // Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
-// -> bottom-up boxes -> two traversal passes (count, then write into
-// scanned per-query segments) with the FP64 predicate.  Each query sorts its
-// own segment by partner id, so the list comes out in canonical order with
-// no global sort and no atomics on the contact list.
+// -> node boxes from leaf ranges -> 64-byte node records (both child boxes,
+// child ids and the last sorted leaf under each child, so a traversal step is
+// one dependent load) -> two traversal passes (count, then write into scanned
+// segments) with the FP64 predicate.
+//
+// Point-triangle: one lane per surface vertex; the vertex owns its segment and
+// sorts it by face id.  Edge-edge: one lane per sorted leaf k, visiting only
+// leaves after k, so each unordered pair is tested once; the pair is counted
+// and stored in the segment of its smaller edge id (integer atomics for the
+// per-segment counts and slots), and every segment is then sorted by the
+// larger id.  The list comes out in canonical order, bit-identical on every
+// run, with no global sort.
 //
 // Reference: lbvh_f/lbvh_e::Construct (src/contact/lbvh/lbvh.cu:1468-1540),
 // _dcd_vf/_dcd_ee (:1694-1813); acceptance follows ProximalQuery
@@ -207,6 +215,23 @@ __global__ void k_node_boxes(int n, const int2* __restrict__ range, const float4
     node_box[2 * i + 1] = hi;
 }
 
+// Node record i (4 float4): (left lo, left id), (left hi, right id),
+// (right lo, last leaf under left), (right hi, last leaf under right); ids and
+// leaf positions are stored as int bits in w.
+__global__ void k_node_records(int n, const int32_t* __restrict__ child, const int2* __restrict__ range,
+                               const float4* __restrict__ node_box, float4* __restrict__ rec) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int l = child[2 * i], r = child[2 * i + 1];
+    const int last_l = l >= n - 1 ? l - (n - 1) : range[l].y;
+    const int last_r = r >= n - 1 ? r - (n - 1) : range[r].y;
+    const float4 llo = node_box[2 * l], lhi = node_box[2 * l + 1], rlo = node_box[2 * r], rhi = node_box[2 * r + 1];
+    rec[4 * i] = make_float4(llo.x, llo.y, llo.z, __int_as_float(l));
+    rec[4 * i + 1] = make_float4(lhi.x, lhi.y, lhi.z, __int_as_float(r));
+    rec[4 * i + 2] = make_float4(rlo.x, rlo.y, rlo.z, __int_as_float(last_l));
+    rec[4 * i + 3] = make_float4(rhi.x, rhi.y, rhi.z, __int_as_float(last_r));
+}
+
 __device__ inline bool box_far(float4 nlo, float4 nhi, float4 qlo, float4 qhi, float gap) {
     return (nlo.x - qhi.x > gap) || (qlo.x - nhi.x > gap) || (nlo.y - qhi.y > gap) || (qlo.y - nhi.y > gap) ||
            (nlo.z - qhi.z > gap) || (qlo.z - nhi.z > gap);
@@ -267,12 +292,11 @@ __device__ inline void sort_segment(const QueryOut& o, int begin, int end) {
 
 // Point-triangle queries: one lane per surface vertex, LDS stack per lane.
 // write == 0: count accepted pairs into qcount; write == 1: store them at
-// qoffset and sort the segment.
+// qoffset and sort the segment by face id.
 __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __restrict__ x, const int32_t* __restrict__ surf_verts,
                                                    int nv, const int32_t* __restrict__ tris, int n_dyn, int nf,
-                                                   const float4* __restrict__ node_box, const int32_t* __restrict__ child,
-                                                   const uint32_t* __restrict__ ids, double dhat, float gap,
-                                                   uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoffset,
+                                                   const float4* __restrict__ rec, const uint32_t* __restrict__ ids, double dhat,
+                                                   float gap, uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoffset,
                                                    QueryOut o, uint32_t* __restrict__ flags) {
     __shared__ int32_t stack[STACK][QBLK];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -287,38 +311,46 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __res
     const int leaf0 = nf - 1;
     int count = 0;
     const int base = write ? (int)qoffset[q] : 0;
-    int sp = 0;
-    int node = 0; // root (a leaf when nf == 1)
-    while (true) {
-        if (node >= leaf0) {
-            const int f = (int)ids[node - leaf0];
-            const int a = tris[3 * f], b = tris[3 * f + 1], c = tris[3 * f + 2];
-            if (v != a && v != b && v != c && !(vstatic && a >= n_dyn)) {
-                const dv3 t0 = ldd(x, a), t1 = ldd(x, b), t2 = ldd(x, c);
-                const PTOut r = point_triangle(p, t0, t1, t2);
-                if (pt_accept(r, dhat)) {
-                    if (write) {
-                        const int idx = base + count;
-                        if (idx < o.cap) write_contact(o, idx, 0, v, f, make_int4(v, a, b, c), pt_fields(p, t0, t1, t2, r, dhat), r.dist);
-                    }
-                    count++;
-                }
-            }
-        } else {
-            const int l = child[2 * node], r = child[2 * node + 1];
-            const bool hl = !box_far(node_box[2 * l], node_box[2 * l + 1], pf, pf, gap);
-            const bool hr = !box_far(node_box[2 * r], node_box[2 * r + 1], pf, pf, gap);
+    auto leaf = [&](int node) {
+        const int f = (int)ids[node - leaf0];
+        const int a = tris[3 * f], b = tris[3 * f + 1], c = tris[3 * f + 2];
+        if (v == a || v == b || v == c || (vstatic && a >= n_dyn)) return;
+        const dv3 t0 = ldd(x, a), t1 = ldd(x, b), t2 = ldd(x, c);
+        const PTOut r = point_triangle(p, t0, t1, t2);
+        if (!pt_accept(r, dhat)) return;
+        if (write) {
+            const int idx = base + count;
+            if (idx < o.cap) write_contact(o, idx, 0, v, f, make_int4(v, a, b, c), pt_fields(p, t0, t1, t2, r, dhat), r.dist);
+        }
+        count++;
+    };
+    if (nf == 1) {
+        leaf(0);
+    } else {
+        int sp = 0, node = 0;
+        while (true) {
+            const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
+            const int l = __float_as_int(a.w), r = __float_as_int(b.w);
+            bool hl = !box_far(a, b, pf, pf, gap), hr = !box_far(c, d, pf, pf, gap);
+            // leaf children are tested here (one call site keeps the FP64 test inlined once)
+            const bool ll = hl && l >= leaf0, lr = hr && r >= leaf0;
+            for (int t = 0; t < 2; t++)
+                if (t == 0 ? ll : lr) leaf(t == 0 ? l : r);
+            hl = hl && !ll;
+            hr = hr && !lr;
             if (hl && hr) {
                 if (sp < STACK) stack[sp++][threadIdx.x] = r;
                 else atomicOr(&flags[1], 1u);
                 node = l;
-                continue;
+            } else if (hl) {
+                node = l;
+            } else if (hr) {
+                node = r;
+            } else {
+                if (sp == 0) break;
+                node = stack[--sp][threadIdx.x];
             }
-            if (hl) { node = l; continue; }
-            if (hr) { node = r; continue; }
         }
-        if (sp == 0) break;
-        node = stack[--sp][threadIdx.x];
     }
     if (!write) {
         qcount[q] = (uint32_t)count;
@@ -328,66 +360,78 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __res
     }
 }
 
-// Edge-edge queries: one lane per surface edge i, partners j > i.
+// Edge-edge queries: lane k takes the edge at sorted leaf k and visits only
+// leaves after k.  A pair (i, j), i < j, belongs to segment i and is evaluated
+// with edge i first, as the canonical definition does.  write == 0: count
+// into qcount[i] (zeroed beforehand); write == 1: store at
+// qoffset[i] + qfill[i]++ (qfill zeroed beforehand; k_sort_segments orders
+// each segment afterwards).
 __global__ void __launch_bounds__(QBLK) k_query_ee(int write, const float* __restrict__ x, const int32_t* __restrict__ edges, int ne,
-                                                   int n_dyn, const float4* __restrict__ node_box,
-                                                   const int32_t* __restrict__ child, const uint32_t* __restrict__ ids,
+                                                   int n_dyn, const float4* __restrict__ rec, const uint32_t* __restrict__ ids,
                                                    double dhat, float gap, uint32_t* __restrict__ qcount,
-                                                   const uint32_t* __restrict__ qoffset, QueryOut o, uint32_t* __restrict__ flags) {
+                                                   const uint32_t* __restrict__ qoffset, uint32_t* __restrict__ qfill, QueryOut o,
+                                                   uint32_t* __restrict__ flags) {
     __shared__ int32_t stack[STACK][QBLK];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ne) return;
-    const int a0 = edges[2 * i], a1 = edges[2 * i + 1];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= ne || ne < 2) return;
+    const int e = (int)ids[k];
+    const int a0 = edges[2 * e], a1 = edges[2 * e + 1];
     const float4 pa0 = make_float4(x[3 * a0], x[3 * a0 + 1], x[3 * a0 + 2], 0.f);
     const float4 pa1 = make_float4(x[3 * a1], x[3 * a1 + 1], x[3 * a1 + 2], 0.f);
     const float4 qlo = fmin4(pa0, pa1), qhi = fmax4(pa0, pa1);
-    const dv3 P0 = dmk(pa0.x, pa0.y, pa0.z), P1 = dmk(pa1.x, pa1.y, pa1.z);
-    const bool astatic = a0 >= n_dyn;
+    const bool estatic = a0 >= n_dyn;
     const int leaf0 = ne - 1;
-    int count = 0;
-    const int base = write ? (int)qoffset[i] : 0;
-    int sp = 0;
-    int node = 0;
-    while (true) {
-        if (node >= leaf0) {
-            const int j = (int)ids[node - leaf0];
-            if (j > i) {
-                const int b0 = edges[2 * j], b1 = edges[2 * j + 1];
-                if (a0 != b0 && a0 != b1 && a1 != b0 && a1 != b1 && !(astatic && b0 >= n_dyn)) {
-                    const dv3 Q0 = ldd(x, b0), Q1 = ldd(x, b1);
-                    const EEOut r = segment_segment(P0, P1, Q0, Q1);
-                    if (ee_accept(r, dhat)) {
-                        if (write) {
-                            const int idx = base + count;
-                            if (idx < o.cap)
-                                write_contact(o, idx, 1, i, j, make_int4(a0, a1, b0, b1), ee_fields(P0, P1, Q0, Q1, r, dhat), r.dist);
-                        }
-                        count++;
-                    }
-                }
-            }
+    auto leaf = [&](int node) {
+        const int f = (int)ids[node - leaf0];
+        const int b0 = edges[2 * f], b1 = edges[2 * f + 1];
+        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1 || (estatic && b0 >= n_dyn)) return;
+        const bool first = e < f;
+        const int i = first ? e : f, j = first ? f : e;
+        const int i0 = first ? a0 : b0, i1 = first ? a1 : b1, j0 = first ? b0 : a0, j1 = first ? b1 : a1;
+        const dv3 P0 = ldd(x, i0), P1 = ldd(x, i1), Q0 = ldd(x, j0), Q1 = ldd(x, j1);
+        const EEOut r = segment_segment(P0, P1, Q0, Q1);
+        if (!ee_accept(r, dhat)) return;
+        if (!write) {
+            atomicAdd(&qcount[i], 1u);
         } else {
-            const int l = child[2 * node], r = child[2 * node + 1];
-            const bool hl = !box_far(node_box[2 * l], node_box[2 * l + 1], qlo, qhi, gap);
-            const bool hr = !box_far(node_box[2 * r], node_box[2 * r + 1], qlo, qhi, gap);
-            if (hl && hr) {
-                if (sp < STACK) stack[sp++][threadIdx.x] = r;
-                else atomicOr(&flags[1], 1u);
-                node = l;
-                continue;
-            }
-            if (hl) { node = l; continue; }
-            if (hr) { node = r; continue; }
+            const int idx = (int)(qoffset[i] + atomicAdd(&qfill[i], 1u));
+            if (idx < o.cap) write_contact(o, idx, 1, i, j, make_int4(i0, i1, j0, j1), ee_fields(P0, P1, Q0, Q1, r, dhat), r.dist);
         }
-        if (sp == 0) break;
-        node = stack[--sp][threadIdx.x];
+    };
+    int sp = 0, node = 0;
+    while (true) {
+        const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
+        const int l = __float_as_int(a.w), r = __float_as_int(b.w);
+        bool hl = __float_as_int(c.w) > k && !box_far(a, b, qlo, qhi, gap);
+        bool hr = __float_as_int(d.w) > k && !box_far(c, d, qlo, qhi, gap);
+        const bool ll = hl && l >= leaf0, lr = hr && r >= leaf0;
+        for (int t = 0; t < 2; t++)
+            if (t == 0 ? ll : lr) leaf(t == 0 ? l : r);
+        hl = hl && !ll;
+        hr = hr && !lr;
+        if (hl && hr) {
+            if (sp < STACK) stack[sp++][threadIdx.x] = r;
+            else atomicOr(&flags[1], 1u);
+            node = l;
+        } else if (hl) {
+            node = l;
+        } else if (hr) {
+            node = r;
+        } else {
+            if (sp == 0) break;
+            node = stack[--sp][threadIdx.x];
+        }
     }
-    if (!write) {
-        qcount[i] = (uint32_t)count;
-    } else {
-        const int end = min(base + count, o.cap);
-        sort_segment(o, base, end);
-    }
+}
+
+// Sorts each of `n` segments [qoffset[s], qoffset[s] + qcount[s]) (clamped to
+// the capacity) by partner id.
+__global__ void k_sort_segments(int n, const uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoffset, QueryOut o) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int begin = (int)qoffset[s];
+    const int end = min(begin + (int)qcount[s], o.cap);
+    if (end - begin > 1) sort_segment(o, begin, end);
 }
 
 } // namespace
@@ -419,6 +463,7 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
         T.ids.alloc(n);
         T.child.alloc(2 * (size_t)std::max(n - 1, 1));
         T.range.alloc(std::max(n - 1, 1));
+        T.rec.alloc(4 * (size_t)std::max(n - 1, 1));
         T.chunk64.alloc(2 * (size_t)((n + 63) / 64));
         T.chunk4k.alloc(2 * (size_t)((n + 4095) / 4096));
         T.scene.alloc(2);
@@ -436,6 +481,7 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
     c.dist.alloc(cap);
     qcount.alloc((size_t)nv + ne + 1);
     qoffset.alloc((size_t)nv + ne + 1);
+    qfill.alloc((size_t)ne + 1);
     total.alloc(2);
     PA_HIP(hipStreamSynchronize(stream));
 }
@@ -475,6 +521,9 @@ void Detector::build_tree(Tree& T, const float* x, bool faces) {
         hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.range.p, leaves, T.chunk64.p, T.chunk4k.p,
                            T.node_box.p);
         PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_node_records, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.child.p, T.range.p, T.node_box.p,
+                           T.rec.p);
+        PA_LAUNCH_CHECK();
     }
 }
 
@@ -484,16 +533,24 @@ int Detector::detect(const float* x) {
     fill<uint32_t>(total.p, 0u, 2, stream);
     const float gap = (float)(dhat * 1.001);
     QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
+    uint32_t* ee_count = qcount.p + nv;
+    const uint32_t* ee_offset = qoffset.p + nv;
     for (int write = 0; write < 2; write++) {
         if (nv > 0)
             hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, write, x, surf_verts.p, nv, tris.p, n_dyn, nf,
-                               ft.node_box.p, ft.child.p, ft.ids.p, dhat, gap, qcount.p, qoffset.p, o, total.p);
+                               ft.rec.p, ft.ids.p, dhat, gap, qcount.p, qoffset.p, o, total.p);
         PA_LAUNCH_CHECK();
-        if (ne > 0)
-            hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, write, x, edges.p, ne, n_dyn, et.node_box.p,
-                               et.child.p, et.ids.p, dhat, gap, qcount.p + nv, qoffset.p + nv, o, total.p);
-        PA_LAUNCH_CHECK();
+        if (ne > 0) {
+            fill<uint32_t>(write ? qfill.p : ee_count, 0u, ne, stream);
+            hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, write, x, edges.p, ne, n_dyn, et.rec.p,
+                               et.ids.p, dhat, gap, ee_count, ee_offset, qfill.p, o, total.p);
+            PA_LAUNCH_CHECK();
+        }
         if (write == 0) scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);
+    }
+    if (ne > 0) {
+        hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(ne, BLK)), dim3(BLK), 0, stream, ne, (const uint32_t*)ee_count, ee_offset, o);
+        PA_LAUNCH_CHECK();
     }
     uint32_t h[2];
     PA_HIP(hipMemcpyAsync(h, total.p, sizeof(h), hipMemcpyDeviceToHost, stream));

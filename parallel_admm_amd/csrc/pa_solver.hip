@@ -30,7 +30,9 @@ namespace {
 constexpr int BLK = 256;
 // Upper bound on the grid of kernels that emit per-block partial sums: every
 // consumer block re-reduces all partials, so their number stays modest.
-constexpr int CG_MAX_BLOCKS = 1024;
+constexpr int CG_MAX_BLOCKS = 256;
+// Block size of every kernel that writes or re-reduces the PCG partial sums.
+constexpr int PBLK = 1024;
 
 // Lanes per CSR row: about three entries per lane, so each lane's dependent
 // gather chain is short and the wave has many rows in flight.
@@ -54,8 +56,9 @@ template <class F> void with_lpr(int lpr, F&& f) {
 // ---------------------------------------------------------------------------
 // Block reduction of K doubles (fixed order -> identical result in every block).
 // ---------------------------------------------------------------------------
-template <int K> __device__ inline void block_sum_write(double (&v)[K], double* __restrict__ part, int stride, int offset) {
-    __shared__ double sh[K][BLK / 64];
+template <int K, int NT = BLK>
+__device__ inline void block_sum_write(double (&v)[K], double* __restrict__ part, int stride, int offset) {
+    __shared__ double sh[K][NT / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; k++) {
@@ -64,25 +67,27 @@ template <int K> __device__ inline void block_sum_write(double (&v)[K], double* 
         if (lane == 0) sh[k][wave] = s;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; k++) {
-            double s = 0;
-            for (int w = 0; w < BLK / 64; w++) s += sh[k][w];
-            part[(size_t)blockIdx.x * stride + offset + k] = s;
-        }
+    // thread k sums component k over the waves (same order in every block)
+    if (threadIdx.x < K) {
+        double s = 0;
+#pragma unroll 1
+        for (int w = 0; w < NT / 64; w++) s += sh[threadIdx.x][w];
+        part[(size_t)blockIdx.x * stride + offset + threadIdx.x] = s;
     }
 }
 
 // Every block sums the same partials in the same order.
-template <int K> __device__ inline void block_reduce_parts(const double* __restrict__ part, int nparts, int stride, int offset,
-                                                           double (&out)[K]) {
-    __shared__ double sh[K][BLK / 64];
+template <int K, int NT = BLK>
+__device__ inline void block_reduce_parts(const double* __restrict__ part, int nparts, int stride, int offset, double (&out)[K]) {
+    __shared__ double sh[K][NT / 64];
+    __shared__ double fin[K];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     double v[K];
 #pragma unroll
     for (int k = 0; k < K; k++) v[k] = 0;
-    for (int b = threadIdx.x; b < nparts; b += BLK)
+    // not unrolled: K accumulators per thread, and nparts <= NT in practice
+#pragma unroll 1
+    for (int b = threadIdx.x; b < nparts; b += NT)
 #pragma unroll
         for (int k = 0; k < K; k++) v[k] += part[(size_t)b * stride + offset + k];
 #pragma unroll
@@ -92,12 +97,15 @@ template <int K> __device__ inline void block_reduce_parts(const double* __restr
         if (lane == 0) sh[k][wave] = s;
     }
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; k++) {
+    if (threadIdx.x < K) {
         double s = 0;
-        for (int w = 0; w < BLK / 64; w++) s += sh[k][w];
-        out[k] = s;
+#pragma unroll 1
+        for (int w = 0; w < NT / 64; w++) s += sh[threadIdx.x][w];
+        fin[threadIdx.x] = s;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) out[k] = fin[k];
     __syncthreads();
 }
 
@@ -598,13 +606,39 @@ template <int LPR> __device__ inline float3 group_sum(float3 s) {
     return s;
 }
 
+// Calls f(val, col) for the entries of row i that lane `sub` of the row group
+// owns (every LPR-th), loading up to RU entries' column ids and values before
+// any gather, so the row costs one chain of dependent loads
+// (ptr -> col/val -> gathered vector) instead of one per entry.  Unused slots
+// pass value 0 and column 0, an index every matrix here has (the row index
+// would not do: the prolongators have more rows than columns).
+constexpr int RU = 4;
+template <int LPR, class F>
+__device__ inline void row_foreach(int i, int rows, int sub, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                   const float* __restrict__ val, F&& f) {
+    if (i >= rows) return;
+    const int e1 = ptr[i + 1];
+    for (int b = ptr[i] + sub; b < e1; b += RU * LPR) {
+        int c[RU];
+        float v[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int e = b + u * LPR;
+            const bool ok = e < e1;
+            c[u] = ok ? col[e] : 0;
+            v[u] = ok ? val[e] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++) f(v[u], c[u]);
+    }
+}
+
 // sum_e val[e] * in[col[e]] over row r (lanes of the group split the entries)
 template <int LPR> __device__ inline float3 row_dot(int r, int rows, int sub, const int32_t* __restrict__ ptr,
                                                     const int32_t* __restrict__ col, const float* __restrict__ val,
                                                     const float* __restrict__ in) {
     float3 s = f3(0.f, 0.f, 0.f);
-    if (r < rows)
-        for (int e = ptr[r] + sub; e < ptr[r + 1]; e += LPR) s = s + val[e] * ld3(in, col[e]);
+    row_foreach<LPR>(r, rows, sub, ptr, col, val, [&](float v, int c) { s = s + v * ld3(in, c); });
     return group_sum<LPR>(s);
 }
 
@@ -613,24 +647,36 @@ template <int LPR> __device__ inline float3 row_dot(int r, int rows, int sub, co
 // correction d starts from d = 0 with r32 = r.  Partials: |b|^2 at 0..2,
 // |r|^2 at 3..5.
 template <int LPR>
-__global__ void __launch_bounds__(BLK) k_cg_resid64(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+__global__ void __launch_bounds__(PBLK) k_cg_resid64(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                     const float* __restrict__ val, const float* __restrict__ val_lo,
                                                     const float* __restrict__ diag, const float* __restrict__ diag_lo,
                                                     const double* __restrict__ bvec, const float* __restrict__ x, float* __restrict__ r,
                                                     float* __restrict__ d, double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
         const int i = r0 + threadIdx.x / LPR;
         double ax[3] = {0, 0, 0};
-        if (i < n)
-            for (int e = ptr[i] + sub; e < ptr[i + 1]; e += LPR) {
-                const double a = (double)val[e] + (double)val_lo[e];
-                const int c = col[e];
-                ax[0] += a * x[3 * c];
-                ax[1] += a * x[3 * c + 1];
-                ax[2] += a * x[3 * c + 2];
+        if (i < n) {
+            const int e1 = ptr[i + 1];
+            for (int b = ptr[i] + sub; b < e1; b += RU * LPR) {
+                int c[RU];
+                double a[RU];
+#pragma unroll
+                for (int u = 0; u < RU; u++) {
+                    const int e = b + u * LPR;
+                    const bool ok = e < e1;
+                    c[u] = ok ? col[e] : 0;
+                    a[u] = ok ? (double)val[e] + (double)val_lo[e] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < RU; u++) {
+                    ax[0] += a[u] * x[3 * c[u]];
+                    ax[1] += a[u] * x[3 * c[u] + 1];
+                    ax[2] += a[u] * x[3 * c[u] + 2];
+                }
             }
+        }
 #pragma unroll
         for (int o = LPR / 2; o > 0; o >>= 1)
 #pragma unroll
@@ -648,11 +694,11 @@ __global__ void __launch_bounds__(BLK) k_cg_resid64(int n, const int32_t* __rest
             }
         }
     }
-    block_sum_write<6>(acc, part, 9, 0);
+    block_sum_write<6, PBLK>(acc, part, 9, 0);
 }
 
 // Jacobi preconditioner z = D^-1 r; partial r.z at 6..8.
-__global__ void __launch_bounds__(BLK) k_jacobi_prec(int n, const float* __restrict__ dinv, const float* __restrict__ r,
+__global__ void __launch_bounds__(PBLK) k_jacobi_prec(int n, const float* __restrict__ dinv, const float* __restrict__ r,
                                                      float* __restrict__ z, double* __restrict__ part) {
     double acc[3] = {0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -664,23 +710,22 @@ __global__ void __launch_bounds__(BLK) k_jacobi_prec(int n, const float* __restr
             acc[k] += (double)rk * zk;
         }
     }
-    block_sum_write<3>(acc, part, 9, 6);
+    block_sum_write<3, PBLK>(acc, part, 9, 6);
 }
 
-// Pass state from the partials: rz, the stop threshold tol^2 |b|^2 (|r_0|^2
-// when b = 0) and |b|^2; a column is done at once when |r_0|^2 is already at or
-// below the threshold.
-__global__ void __launch_bounds__(BLK) k_cg_init_state(const double* __restrict__ part, int nparts, double tol2,
+// Pass state from the partials: rz, |b|^2 and the stop threshold
+// tol^2 |r_0|^2 per column (|b|^2 when r_0 = 0); a column whose r_0 is zero
+// is done at once.
+__global__ void __launch_bounds__(PBLK) k_cg_init_state(const double* __restrict__ part, int nparts, double tol2,
                                                        double* __restrict__ st) {
     double s[9];
-    block_reduce_parts<9>(part, nparts, 9, 0, s);
+    block_reduce_parts<9, PBLK>(part, nparts, 9, 0, s);
     if (threadIdx.x == 0) {
         for (int k = 0; k < 3; k++) {
-            const double thr = tol2 * (s[k] > 0.0 ? s[k] : s[3 + k]);
             st[k] = s[6 + k];
-            st[16 + k] = thr;
+            st[16 + k] = tol2 * s[3 + k];
             st[20 + k] = s[k];
-            st[3 + k] = (s[3 + k] <= thr || s[3 + k] == 0.0) ? 1.0 : 0.0;
+            st[3 + k] = (s[3 + k] <= 1e-24 * s[k] || s[3 + k] == 0.0) ? 1.0 : 0.0;
         }
         st[19] = 0;
     }
@@ -699,12 +744,12 @@ __global__ void k_axpy_d(int n, const float* __restrict__ d, float* __restrict__
 
 // q = A pd (off-diagonal CSR + diagonal); partial p.q at 0..2
 template <int LPR>
-__global__ void __launch_bounds__(BLK) k_cg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+__global__ void __launch_bounds__(PBLK) k_cg_spmv(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                  const float* __restrict__ val, const float* __restrict__ diag,
                                                  const float* __restrict__ pd, float* __restrict__ q, double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
         const int i = r0 + threadIdx.x / LPR;
         float3 s = row_dot<LPR>(i, n, sub, ptr, col, val, pd);
         if (i < n && sub == 0) {
@@ -716,15 +761,15 @@ __global__ void __launch_bounds__(BLK) k_cg_spmv(int n, const int32_t* __restric
             acc[2] += (double)pi.z * s.z;
         }
     }
-    block_sum_write<3>(acc, part, 9, 0);
+    block_sum_write<3, PBLK>(acc, part, 9, 0);
 }
 
 // alpha = rz / pq; d += alpha pd; r -= alpha q; partial |r|^2 at 3..5
-__global__ void __launch_bounds__(BLK) k_cg_update(int n, int slot, int nparts, const double* __restrict__ st_in,
+__global__ void __launch_bounds__(PBLK) k_cg_update(int n, int slot, int nparts, const double* __restrict__ st_in,
                                                    const float* __restrict__ pd, const float* __restrict__ q, float* __restrict__ x,
                                                    float* __restrict__ r, double* __restrict__ part) {
     double pq[3];
-    block_reduce_parts<3>(part, nparts, 9, 0, pq);
+    block_reduce_parts<3, PBLK>(part, nparts, 9, 0, pq);
     const double* s = st_in + slot * 8;
     float al[3];
     for (int k = 0; k < 3; k++) al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
@@ -740,15 +785,15 @@ __global__ void __launch_bounds__(BLK) k_cg_update(int n, int slot, int nparts, 
         acc[1] += (double)ri.y * ri.y;
         acc[2] += (double)ri.z * ri.z;
     }
-    block_sum_write<3>(acc, part, 9, 3);
+    block_sum_write<3, PBLK>(acc, part, 9, 3);
 }
 
 // beta = rz_new / rz_old (rz at 6..8), done when |r|^2 (3..5) <= threshold;
 // pd = zc + beta pd; block 0 advances the state slot.
-__global__ void __launch_bounds__(BLK) k_cg_dir(int n, int slot, int nparts, double* __restrict__ st,
+__global__ void __launch_bounds__(PBLK) k_cg_dir(int n, int slot, int nparts, double* __restrict__ st,
                                                 const float* __restrict__ zc, float* __restrict__ pd, const double* __restrict__ part) {
     double s6[6];
-    block_reduce_parts<6>(part, nparts, 9, 3, s6); // s6[0..2] = |r|^2, s6[3..5] = r.z
+    block_reduce_parts<6, PBLK>(part, nparts, 9, 3, s6); // s6[0..2] = |r|^2, s6[3..5] = r.z
     const double* so = st + slot * 8;
     float be[3];
     double done[3];
@@ -773,6 +818,126 @@ __global__ void __launch_bounds__(BLK) k_cg_dir(int n, int slot, int nparts, dou
 }
 
 // ---------------------------------------------------------------------------
+// Fused PCG iteration for the AMG preconditioner (two kernels plus the rest of
+// the V-cycle instead of four plus a full V-cycle):
+//   k_cg_dir_spmv:   beta from the previous iteration's partials, then
+//                    p' = z + beta p and q = A p' with p' formed on the fly for
+//                    every gathered neighbour (p' goes to the other p buffer);
+//   k_cg_update_pre: alpha = rz / p'.q, d += alpha p', r' = r - alpha q, and the
+//                    level-0 pre-smoothing of r' (again on the fly from r and q;
+//                    r' goes to the other r buffer).
+// ---------------------------------------------------------------------------
+// Next state slot (slot_old ^ 1) from |r|^2 at 3..5 and r.z at 6..8; every
+// thread gets beta, block 0 thread 0 writes the slot.
+__device__ inline void cg_advance(const double* __restrict__ part, int nparts, double* __restrict__ st, int slot_old, float (&be)[3]) {
+    double s6[6];
+    block_reduce_parts<6, PBLK>(part, nparts, 9, 3, s6);
+    const double* so = st + slot_old * 8;
+    double done[3];
+    for (int k = 0; k < 3; k++) {
+        be[k] = (so[k] == 0.0) ? 0.f : (float)(s6[3 + k] / so[k]);
+        done[k] = (so[3 + k] != 0.0 || s6[k] <= st[16 + k]) ? 1.0 : 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        double* sn = st + (slot_old ^ 1) * 8;
+        for (int k = 0; k < 3; k++) {
+            sn[k] = s6[3 + k];
+            sn[3 + k] = done[k];
+        }
+    }
+}
+
+// State update alone (before a convergence read-back); idempotent with the
+// one k_cg_dir_spmv does at the start of the next iteration.
+__global__ void __launch_bounds__(PBLK) k_cg_state(int slot_old, int nparts, double* __restrict__ st, const double* __restrict__ part) {
+    float be[3];
+    cg_advance(part, nparts, st, slot_old, be);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot_old, int nparts, double* __restrict__ st,
+                                                     const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                     const float* __restrict__ val, const float* __restrict__ diag,
+                                                     const float* __restrict__ z, const float* __restrict__ p_old,
+                                                     float* __restrict__ p_new, float* __restrict__ q, double* __restrict__ part) {
+    float be[3] = {0.f, 0.f, 0.f};
+    if (!first) cg_advance(part, nparts, st, slot_old, be);
+    const int sub = threadIdx.x % LPR;
+    double acc[3] = {0, 0, 0};
+    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
+        const int i = r0 + threadIdx.x / LPR;
+        float3 s = f3(0.f, 0.f, 0.f);
+        if (first) {
+            row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) { s = s + v * ld3(z, c); });
+        } else {
+            row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) {
+                const float3 zc = ld3(z, c), pc = ld3(p_old, c);
+                s = s + v * f3(zc.x + be[0] * pc.x, zc.y + be[1] * pc.y, zc.z + be[2] * pc.z);
+            });
+        }
+        s = group_sum<LPR>(s);
+        if (i < n && sub == 0) {
+            const float3 zi = ld3(z, i);
+            const float3 pi = first ? zi : [&] {
+                const float3 po = ld3(p_old, i);
+                return f3(zi.x + be[0] * po.x, zi.y + be[1] * po.y, zi.z + be[2] * po.z);
+            }();
+            s = s + diag[i] * pi;
+            st3(p_new, i, pi);
+            st3(q, i, s);
+            acc[0] += (double)pi.x * s.x;
+            acc[1] += (double)pi.y * s.y;
+            acc[2] += (double)pi.z * s.z;
+        }
+    }
+    block_sum_write<3, PBLK>(acc, part, 9, 0);
+}
+
+template <int LPR>
+__global__ void __launch_bounds__(PBLK) k_cg_update_pre(int n, int slot, int nparts, double* __restrict__ st,
+                                                       const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                                       const float* __restrict__ val, const float* __restrict__ dinv, float w,
+                                                       const float* __restrict__ pd, const float* __restrict__ q,
+                                                       float* __restrict__ d, const float* __restrict__ r_old,
+                                                       float* __restrict__ r_new, float* __restrict__ x0, float* __restrict__ res,
+                                                       double* __restrict__ part) {
+    double pq[3];
+    block_reduce_parts<3, PBLK>(part, nparts, 9, 0, pq);
+    const double* s = st + slot * 8;
+    float al[3];
+    bool all_done = true;
+    for (int k = 0; k < 3; k++) {
+        al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
+        all_done = all_done && s[3 + k] != 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !all_done) st[19] += 1.0;
+    const int sub = threadIdx.x % LPR;
+    double acc[3] = {0, 0, 0};
+    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
+        const int i = r0 + threadIdx.x / LPR;
+        float3 sm = f3(0.f, 0.f, 0.f);
+        row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) {
+            const float3 rc = ld3(r_old, c), qc = ld3(q, c);
+            const float3 rn = f3(rc.x - al[0] * qc.x, rc.y - al[1] * qc.y, rc.z - al[2] * qc.z);
+            sm = sm + (v * (w * dinv[c])) * rn;
+        });
+        sm = group_sum<LPR>(sm);
+        if (i < n && sub == 0) {
+            const float3 ri = ld3(r_old, i), qi = ld3(q, i), pi = ld3(pd, i), di = ld3(d, i);
+            const float3 rn = f3(ri.x - al[0] * qi.x, ri.y - al[1] * qi.y, ri.z - al[2] * qi.z);
+            st3(d, i, f3(di.x + al[0] * pi.x, di.y + al[1] * pi.y, di.z + al[2] * pi.z));
+            st3(r_new, i, rn);
+            st3(x0, i, (w * dinv[i]) * rn);
+            st3(res, i, rn - sm);
+            acc[0] += (double)rn.x * rn.x;
+            acc[1] += (double)rn.y * rn.y;
+            acc[2] += (double)rn.z * rn.z;
+        }
+    }
+    block_sum_write<3, PBLK>(acc, part, 9, 3);
+}
+
+// ---------------------------------------------------------------------------
 // Smoothed-aggregation AMG V-cycle (one weighted-Jacobi pre- and post-sweep per
 // level, dense inverse on the coarsest level); symmetric, so usable as the PCG
 // preconditioner.  Level matrices hold the diagonal.
@@ -786,11 +951,7 @@ __global__ void __launch_bounds__(BLK) k_amg_pre(int n, const int32_t* __restric
     const int sub = threadIdx.x % LPR;
     const int i = blockIdx.x * (BLK / LPR) + threadIdx.x / LPR;
     float3 s = f3(0.f, 0.f, 0.f);
-    if (i < n)
-        for (int e = ptr[i] + sub; e < ptr[i + 1]; e += LPR) {
-            const int c = col[e];
-            s = s + (val[e] * (w * dinv[c])) * ld3(b, c);
-        }
+    row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) { s = s + (v * (w * dinv[c])) * ld3(b, c); });
     s = group_sum<LPR>(s);
     if (i < n && sub == 0) {
         const float3 bi = ld3(b, i);
@@ -813,13 +974,13 @@ __global__ void __launch_bounds__(BLK) k_amg_spmv(int rows, const int32_t* __res
 // Post-smoothing: xo = xi + w D^-1 (b - A xi); with part != null also b.xo at
 // 6..8 (on the finest level b is the PCG residual, so this is r.z).
 template <int LPR>
-__global__ void __launch_bounds__(BLK) k_amg_post(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+__global__ void __launch_bounds__(PBLK) k_amg_post(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                   const float* __restrict__ val, const float* __restrict__ dinv, float w,
                                                   const float* __restrict__ b, const float* __restrict__ xi, float* __restrict__ xo,
                                                   double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (BLK / LPR); r0 < n; r0 += gridDim.x * (BLK / LPR)) {
+    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
         const int i = r0 + threadIdx.x / LPR;
         const float3 ax = row_dot<LPR>(i, n, sub, ptr, col, val, xi);
         if (i < n && sub == 0) {
@@ -831,35 +992,38 @@ __global__ void __launch_bounds__(BLK) k_amg_post(int n, const int32_t* __restri
             acc[2] += (double)bi.z * v.z;
         }
     }
-    if (part) block_sum_write<3>(acc, part, 9, 6);
+    if (part) block_sum_write<3, PBLK>(acc, part, 9, 6);
 }
 
 // Partial r.z at 6..8.
-__global__ void __launch_bounds__(BLK) k_rz_part(int n, const float* __restrict__ r, const float* __restrict__ z,
+__global__ void __launch_bounds__(PBLK) k_rz_part(int n, const float* __restrict__ r, const float* __restrict__ z,
                                                  double* __restrict__ part) {
     double acc[3] = {0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         for (int k = 0; k < 3; k++) acc[k] += (double)r[3 * i + k] * z[3 * i + k];
-    block_sum_write<3>(acc, part, 9, 6);
+    block_sum_write<3, PBLK>(acc, part, 9, 6);
 }
 
-// Coarsest level: x = A_c^-1 b with the dense inverse (nc <= 2048), one wave
-// per row reading the row with coalesced loads; b is staged in LDS.
-__global__ void __launch_bounds__(BLK) k_amg_coarse(int nc, const float* __restrict__ inv, const float* __restrict__ b,
+// Coarsest level: x = A_c^-1 b with the dense inverse (nc <= 2048, rows
+// padded with zeros to ncp, a multiple of 4, so every lane reads float4), one
+// wave per row; b is staged in LDS (zero beyond nc).
+__global__ void __launch_bounds__(BLK) k_amg_coarse(int nc, int ncp, const float4* __restrict__ inv, const float* __restrict__ b,
                                                     float* __restrict__ x) {
     __shared__ float sb[3 * 2048];
-    for (int j = threadIdx.x; j < 3 * nc; j += BLK) sb[j] = b[j];
+    for (int j = threadIdx.x; j < 3 * ncp; j += BLK) sb[j] = j < 3 * nc ? b[j] : 0.f;
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int row = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
     if (row >= nc) return;
-    const float* rp = inv + (size_t)row * nc;
+    const float4* rp = inv + (size_t)row * (ncp / 4);
     float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-    for (int j = lane; j < nc; j += 64) {
-        const float a = rp[j];
-        s0 += a * sb[3 * j];
-        s1 += a * sb[3 * j + 1];
-        s2 += a * sb[3 * j + 2];
+#pragma unroll 4
+    for (int j4 = lane; j4 < ncp / 4; j4 += 64) {
+        const float4 a = rp[j4];
+        const float* bb = sb + 12 * j4;
+        s0 += a.x * bb[0] + a.y * bb[3] + a.z * bb[6] + a.w * bb[9];
+        s1 += a.x * bb[1] + a.y * bb[4] + a.z * bb[7] + a.w * bb[10];
+        s2 += a.x * bb[2] + a.y * bb[5] + a.z * bb[8] + a.w * bb[11];
     }
     for (int o = 32; o > 0; o >>= 1) {
         s0 += __shfl_xor(s0, o, 64);
@@ -1133,8 +1297,17 @@ void Backend::assemble_and_upload(const double* x0h) {
             ok = false;
         }
         if (ok && H.coarse_n <= 2048) {
+            // the device kernels index vectors by these columns: check them once
+            auto cols_ok = [](const HostCsr& M) {
+                if ((int)M.ptr.size() != M.rows + 1 || M.ptr[M.rows] != (int32_t)M.col.size()) return false;
+                for (int32_t c : M.col)
+                    if (c < 0 || c >= M.cols) return false;
+                return M.cols > 0;
+            };
             for (size_t l = 0; l < H.levels.size(); l++) {
                 const AmgLevel& L = H.levels[l];
+                PA_CHECK(cols_ok(L.A) && (l + 1 == H.levels.size() || (cols_ok(L.P) && cols_ok(L.R))), PA_ERR_INVALID,
+                         "AMG level with column indices out of range");
                 auto D = std::make_unique<DevLevel>();
                 D->n = L.A.rows;
                 D->omega = (float)L.omega;
@@ -1163,9 +1336,12 @@ void Backend::assemble_and_upload(const double* x0h) {
                 D->r.alloc(3 * (size_t)D->n);
                 amg.push_back(std::move(D));
             }
-            std::vector<float> ci(H.coarse_inv.begin(), H.coarse_inv.end());
-            amg_cinv.upload(ci, stream);
             amg_nc = H.coarse_n;
+            amg_ncp = (amg_nc + 3) & ~3;
+            std::vector<float> ci((size_t)amg_nc * amg_ncp, 0.f);
+            for (int i = 0; i < amg_nc; i++)
+                for (int j = 0; j < amg_nc; j++) ci[(size_t)i * amg_ncp + j] = (float)H.coarse_inv[(size_t)i * amg_nc + j];
+            amg_cinv.upload(ci, stream);
             use_amg = true;
         }
     }
@@ -1271,8 +1447,10 @@ void Backend::assemble_and_upload(const double* x0h) {
     cg_p.alloc(3 * (size_t)n);
     cg_q.alloc(3 * (size_t)n);
     cg_d.alloc(3 * (size_t)n);
+    cg_r2.alloc(3 * (size_t)n);
+    cg_p2.alloc(3 * (size_t)n);
     a_lpr = lanes_per_row(nnz_a_off + n, n);
-    cg_blocks = std::max(1, std::min(grid_for((int64_t)n * a_lpr, BLK), CG_MAX_BLOCKS));
+    cg_blocks = std::max(1, std::min(grid_for((int64_t)n * a_lpr, PBLK), CG_MAX_BLOCKS));
     cg_part.alloc(9 * (size_t)cg_blocks);
     cg_state.alloc(24);
     int elem_blocks = grid_for(hc.n_tri, BLK) + grid_for(hc.n_tet, BLK) + grid_for(hc.n_bend, BLK) + grid_for(hc.n_pin, BLK);
@@ -1410,14 +1588,13 @@ hipEvent_t Backend::next_event() {
     return ev_pool[ev_next++];
 }
 
-// Mixed-precision refinement: each pass computes the FP64 residual of the
-// FP32 iterate, solves A d = r with FP32 PCG to |r_k| <= global_tol |r_0|,
-// and adds d.  Passes stop early once |r| <= 1e-12 |b| (FP64).  Returns the
-// PCG iterations summed over passes.
-void Backend::precondition() {
+// z = P^-1 r for the PCG: Jacobi, or one AMG V-cycle (downward pre-smoothing
+// and restriction, dense coarse solve, upward prolongation and
+// post-smoothing).  Both variants leave the r.z partials at 6..8 of cg_part.
+void Backend::precondition(const float* r, bool skip_pre0) {
     const int nb = cg_blocks;
     if (!use_amg) {
-        hipLaunchKernelGGL(k_jacobi_prec, dim3(nb), dim3(BLK), 0, stream, n, a_dinv.p, cg_r.p, cg_z.p, cg_part.p);
+        hipLaunchKernelGGL(k_jacobi_prec, dim3(nb), dim3(PBLK), 0, stream, n, a_dinv.p, r, cg_z.p, cg_part.p);
         PA_LAUNCH_CHECK();
         return;
     }
@@ -1426,13 +1603,15 @@ void Backend::precondition() {
     for (int l = 0; l + 1 < nl; l++) {
         DevLevel& L = *amg[l];
         DevLevel& C = *amg[l + 1];
-        const float* bl = l == 0 ? cg_r.p : L.b.p;
-        with_lpr(L.lpr_a, [&](auto c) {
-            constexpr int P = decltype(c)::value;
-            hipLaunchKernelGGL(k_amg_pre<P>, dim3(grid_for((int64_t)L.n * P, BLK)), dim3(BLK), 0, stream, L.n, L.aptr.p, L.acol.p,
-                               L.aval.p, L.dinv.p, L.omega, bl, L.x.p, L.r.p);
-        });
-        PA_LAUNCH_CHECK();
+        const float* bl = l == 0 ? r : L.b.p;
+        if (!(l == 0 && skip_pre0)) {
+            with_lpr(L.lpr_a, [&](auto c) {
+                constexpr int P = decltype(c)::value;
+                hipLaunchKernelGGL(k_amg_pre<P>, dim3(grid_for((int64_t)L.n * P, BLK)), dim3(BLK), 0, stream, L.n, L.aptr.p, L.acol.p,
+                                   L.aval.p, L.dinv.p, L.omega, bl, L.x.p, L.r.p);
+            });
+            PA_LAUNCH_CHECK();
+        }
         with_lpr(L.lpr_r, [&](auto c) {
             constexpr int P = decltype(c)::value;
             hipLaunchKernelGGL(k_amg_spmv<P>, dim3(grid_for((int64_t)L.nnext * P, BLK)), dim3(BLK), 0, stream, L.nnext, L.rptr.p,
@@ -1441,19 +1620,20 @@ void Backend::precondition() {
         PA_LAUNCH_CHECK();
     }
     DevLevel& Cl = *amg[nl - 1];
-    const float* bc = nl == 1 ? cg_r.p : Cl.b.p;
+    const float* bc = nl == 1 ? r : Cl.b.p;
     float* xc = nl == 1 ? cg_z.p : Cl.x2.p;
-    hipLaunchKernelGGL(k_amg_coarse, dim3(grid_for(amg_nc, BLK / 64)), dim3(BLK), 0, stream, amg_nc, amg_cinv.p, bc, xc);
+    hipLaunchKernelGGL(k_amg_coarse, dim3(grid_for(amg_nc, BLK / 64)), dim3(BLK), 0, stream, amg_nc, amg_ncp,
+                       (const float4*)amg_cinv.p, bc, xc);
     PA_LAUNCH_CHECK();
     if (nl == 1) { // the whole system is the coarse level: exact solve, then r.z partials
-        hipLaunchKernelGGL(k_rz_part, dim3(nb), dim3(BLK), 0, stream, n, cg_r.p, cg_z.p, cg_part.p);
+        hipLaunchKernelGGL(k_rz_part, dim3(nb), dim3(PBLK), 0, stream, n, r, cg_z.p, cg_part.p);
         PA_LAUNCH_CHECK();
         return;
     }
     // upward: x_l += P x_{l+1}, then post-smooth into x2_l (cg_z on level 0)
     for (int l = nl - 2; l >= 0; l--) {
         DevLevel& L = *amg[l];
-        const float* bl = l == 0 ? cg_r.p : L.b.p;
+        const float* bl = l == 0 ? r : L.b.p;
         with_lpr(L.lpr_p, [&](auto c) {
             constexpr int P = decltype(c)::value;
             hipLaunchKernelGGL(k_amg_spmv<P>, dim3(grid_for((int64_t)L.n * P, BLK)), dim3(BLK), 0, stream, L.n, L.pptr.p, L.pcol.p,
@@ -1464,8 +1644,8 @@ void Backend::precondition() {
         // level 0 emits r.z partials, so it runs on the partial-sum grid
         with_lpr(L.lpr_a, [&](auto c) {
             constexpr int P = decltype(c)::value;
-            const int grid = l == 0 ? nb : grid_for((int64_t)L.n * P, BLK);
-            hipLaunchKernelGGL(k_amg_post<P>, dim3(grid), dim3(BLK), 0, stream, L.n, L.aptr.p, L.acol.p, L.aval.p, L.dinv.p, L.omega,
+            const int grid = l == 0 ? nb : grid_for((int64_t)L.n * P, PBLK);
+            hipLaunchKernelGGL(k_amg_post<P>, dim3(grid), dim3(PBLK), 0, stream, L.n, L.aptr.p, L.acol.p, L.aval.p, L.dinv.p, L.omega,
                                bl, (const float*)L.x.p, out, l == 0 ? cg_part.p : (double*)nullptr);
         });
         PA_LAUNCH_CHECK();
@@ -1473,35 +1653,42 @@ void Backend::precondition() {
 }
 
 // Mixed-precision refinement: each pass computes the FP64 residual
-// r = b - A x of the FP32 iterate, solves A d = r with preconditioned FP32 CG
-// until |r_k| <= tol |b| per column (the recursive residual), and adds d.  A
-// later pass runs only while the FP64 residual is still above tol |b|, which
-// catches the drift between recursive and true residual.  Iterations are
+// r_0 = b - A x of the FP32 iterate, solves A d = r_0 with preconditioned FP32
+// CG until |r_k| <= tol |r_0| per column (the recursive residual), and adds d.
+// A later pass runs only while the FP64 residual exceeds 1e-12 |b|.  Iterations are
 // launched in chunks between convergence read-backs; the first chunk is sized
 // by the previous solve's count (consecutive ADMM iterations need similar
 // counts).  Returns the PCG iterations summed over passes.
 int Backend::global_solve(int max_iter, double tol) {
     const int nb = cg_blocks;
     const int passes = std::max(1, cfg.global_refine);
+    // the fused iteration needs a real level 0 (nl >= 2); a one-level
+    // hierarchy is an exact coarse solve and keeps the plain sequence
+    const bool fused = use_amg && amg.size() >= 2;
     int total_iters = 0;
     double hs[24];
     for (int pass = 0; pass < passes; pass++) {
         with_lpr(a_lpr, [&](auto c) {
             constexpr int P = decltype(c)::value;
-            hipLaunchKernelGGL(k_cg_resid64<P>, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p,
+            hipLaunchKernelGGL(k_cg_resid64<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p,
                                a_diag_lo.p, b.p, x.p, cg_r.p, cg_d.p, cg_part.p);
         });
         PA_LAUNCH_CHECK();
-        precondition();
-        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(BLK), 0, stream, cg_part.p, nb, tol * tol, cg_state.p);
+        precondition(cg_r.p);
+        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(PBLK), 0, stream, cg_part.p, nb, tol * tol, cg_state.p);
         PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_copy, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, 3 * (int64_t)n, cg_z.p, cg_p.p);
-        PA_LAUNCH_CHECK();
+        if (!fused) {
+            hipLaunchKernelGGL(k_copy, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, 3 * (int64_t)n, cg_z.p, cg_p.p);
+            PA_LAUNCH_CHECK();
+        }
         if (pass > 0) {
             PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
             PA_HIP(hipStreamSynchronize(stream));
             if (hs[3] != 0 && hs[4] != 0 && hs[5] != 0) break;
         }
+        float* pbuf[2] = {cg_p.p, cg_p2.p};
+        float* rbuf[2] = {cg_r.p, cg_r2.p};
+        int pc = 0, rc_ = 0;
         int chunk = std::max(2, pass == 0 ? last_pcg_iters : 2);
         int it = 0;
         while (it < max_iter) {
@@ -1515,21 +1702,48 @@ int Backend::global_solve(int max_iter, double tol) {
                     e1 = next_event();
                     PA_HIP(hipEventRecord(e0, stream));
                 }
-                with_lpr(a_lpr, [&](auto c) {
-                    constexpr int P = decltype(c)::value;
-                    hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(BLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
-                                       (const float*)cg_p.p, cg_q.p, cg_part.p);
-                });
+                if (fused) {
+                    with_lpr(a_lpr, [&](auto c) {
+                        constexpr int P = decltype(c)::value;
+                        hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, it == 0 ? 1 : 0, slot ^ 1, nb, cg_state.p,
+                                           a_ptr.p, a_col.p, a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)pbuf[pc],
+                                           pbuf[pc ^ 1], cg_q.p, cg_part.p);
+                    });
+                    pc ^= 1;
+                } else {
+                    with_lpr(a_lpr, [&](auto c) {
+                        constexpr int P = decltype(c)::value;
+                        hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
+                                           (const float*)cg_p.p, cg_q.p, cg_part.p);
+                    });
+                }
                 PA_LAUNCH_CHECK();
                 if (profiling & 1) {
                     PA_HIP(hipEventRecord(e1, stream));
                     spmv_marks.push_back({e0, e1});
                 }
-                hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, cg_p.p, cg_q.p, cg_d.p, cg_r.p,
-                                   cg_part.p);
-                PA_LAUNCH_CHECK();
-                precondition();
-                hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(BLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
+                if (fused) {
+                    DevLevel& L0 = *amg[0];
+                    with_lpr(L0.lpr_a, [&](auto c) {
+                        constexpr int P = decltype(c)::value;
+                        hipLaunchKernelGGL(k_cg_update_pre<P>, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, L0.aptr.p,
+                                           L0.acol.p, L0.aval.p, L0.dinv.p, L0.omega, (const float*)pbuf[pc], (const float*)cg_q.p,
+                                           cg_d.p, (const float*)rbuf[rc_], rbuf[rc_ ^ 1], L0.x.p, L0.r.p, cg_part.p);
+                    });
+                    PA_LAUNCH_CHECK();
+                    rc_ ^= 1;
+                    precondition(rbuf[rc_], true);
+                } else {
+                    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_p.p, cg_q.p, cg_d.p,
+                                       cg_r.p, cg_part.p);
+                    PA_LAUNCH_CHECK();
+                    precondition(cg_r.p);
+                    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
+                    PA_LAUNCH_CHECK();
+                }
+            }
+            if (fused) { // the state slot of iteration `it`, which k_cg_dir_spmv would compute next
+                hipLaunchKernelGGL(k_cg_state, dim3(1), dim3(PBLK), 0, stream, (it - 1) & 1, nb, cg_state.p, (const double*)cg_part.p);
                 PA_LAUNCH_CHECK();
             }
             PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));

@@ -1,0 +1,78 @@
+"""Writes the golden fixtures under tests/golden/.
+
+1. reference_grids.npz: the regular grid meshes that ship with the reference
+   (resource/flat/grid_5x5.obj, grid_20x20.obj and resource/mini/cloth3x3.obj),
+   parsed from their OBJ text into vertex and 0-based face arrays.  These are
+   the reference's own input data; they pin the grid layout (vertex order and
+   the alternating-diagonal triangulation) of this build's cloth generator.
+2. cloth32_trajectory.npz: the FP64 oracle's 60-frame run of the 32x32 cloth
+   workload (dynamic vertex positions at the frames in KEEP, contacts and ADMM
+   iterations of every frame), a regression vector for the GPU path that needs
+   no oracle run.
+
+Run from the repository root in the container that holds the reference:
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REFERENCE = "/root/reference"
+FRAMES = 60                                  # the 32x32 cloth workload runs 60 frames
+KEEP = (1, 2, 3, 10, 20, 30, 40, 50, 60)     # frames stored in the fixture
+
+
+def read_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
+    verts, faces = [], []
+    with open(path) as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                verts.append([float(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                faces.append([int(t.split("/")[0]) - 1 for t in tok[1:4]])
+    return np.array(verts, np.float64), np.array(faces, np.int32)
+
+
+def grids() -> None:
+    out = {}
+    for key, rel in (("grid5", "resource/flat/grid_5x5.obj"), ("grid20", "resource/flat/grid_20x20.obj"),
+                     ("cloth3", "resource/mini/cloth3x3.obj")):
+        v, f = read_obj(os.path.join(REFERENCE, rel))
+        out[f"{key}_v"] = v
+        out[f"{key}_f"] = f
+    np.savez_compressed(os.path.join(HERE, "reference_grids.npz"), **out)
+
+
+def trajectory() -> None:
+    sys.path.insert(0, ROOT)
+    import parallel_admm_amd as pa
+    from oracle.pyoracle import Oracle
+    from parallel_admm_amd.configs import config_for
+
+    scene = pa.Scene.generate(1)
+    cfg = config_for("cloth32")
+    o = Oracle(scene.mesh(), cfg)
+    n = scene.info["n_dyn"]
+    xs, ncs, its = [], [], []
+    for f in range(1, FRAMES + 1):
+        st = o.step()
+        if f in KEEP:
+            xs.append(o.state()[0][:n].copy())
+        ncs.append(st["n_contacts"])
+        its.append(st["iterations"])
+    np.savez_compressed(os.path.join(HERE, "cloth32_trajectory.npz"), frames=np.array(KEEP), x=np.array(xs),
+                        n_contacts=np.array(ncs), iterations=np.array(its), admm_max_iter=cfg.admm_max_iter)
+
+
+if __name__ == "__main__":
+    grids()
+    trajectory()
+    print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
