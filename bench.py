@@ -1,8 +1,9 @@
 """Benchmark: simulation steps per second of the ADMM contact step on one MI355X,
 run to ADMM residual <= 1e-4 (capped at admm_max_iter), with the roofline of
-the dominant kernel and the FP64 CPU oracle timed on the host beside it.
+the PCG SpMV kernel and the FP64 CPU oracle timed on the host beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cloth256|tetbox|cloth32]
+    python bench.py --copy-calibration     # plain device copy of known size (for rocprofv3 --pmc scaling)
 
 Prints one JSON line.  A "step" is one simulation time step of the whole scene
 (detection, local/contact/global ADMM iterations) on synthetic geometry with
@@ -13,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,6 +24,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+ROOFLINE_KERNEL = "k_cg_dir_spmv"
+COPY_BYTES = 1 << 28   # calibration copy: 256 MiB read + 256 MiB written per launch
+COPY_LAUNCHES = 20
 
 
 def spmv_bytes(n: int, nnz_off: int) -> int:
@@ -33,25 +38,56 @@ def spmv_bytes(n: int, nnz_off: int) -> int:
     return 8 * nnz_off + 4 * (n + 1) + 4 * n + 24 * n + 24 * n
 
 
-def cpu_baseline(workload: str, scene, cfg, seconds: float = 20.0) -> dict:
-    """FP64 oracle (prefactored sparse LLT + OpenMP local step) on the host cores,
-    bounded to about `seconds` of stepping; returns steps/s."""
+def measured_traffic(workload: str) -> tuple[int | None, str | None]:
+    """HBM bytes per launch of the roofline kernel from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel
+    factor), if one exists for this workload."""
+    path = os.path.join(ROOT, "profiles", f"pmc_{ROOFLINE_KERNEL}_{workload}.md")
+    if not os.path.exists(path):
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9]+)", open(path).read())
+    return (int(m.group(1)), os.path.relpath(path, ROOT)) if m else (None, None)
+
+
+def copy_calibration() -> int:
+    """A plain device-to-device copy of COPY_BYTES, launched COPY_LAUNCHES
+    times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch give
+    the counters' scale against a known byte count."""
+    import torch
+
+    a = torch.empty(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0").uniform_()
+    b = torch.empty_like(a)
+    for _ in range(COPY_LAUNCHES):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": COPY_BYTES, "copy_bytes_written": COPY_BYTES, "launches": COPY_LAUNCHES}))
+    return 0
+
+
+def cpu_baseline(workload: str, scene, cfg, state, seconds: float) -> dict:
+    """FP64 oracle (prefactored sparse LLT, OpenMP local step and broad phase)
+    on the host cores, started from the GPU's state at the beginning of its
+    timed region and run for about `seconds` (at least one step)."""
     from oracle.pyoracle import Oracle
 
     cores = os.cpu_count() or 1
-    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
     o = Oracle(scene.mesh(), cfg)
-    o.step()  # warm-up frame
+    x, v, ue, uc = state
+    o.set_state(x, v)
+    o.set_duals(ue, uc)
     t0 = time.perf_counter()
-    steps = 0
+    steps, iters = 0, 0
     while True:
-        o.step()
+        st = o.step()
         steps += 1
-        if time.perf_counter() - t0 > seconds or steps >= 200:
+        iters += st["iterations"]
+        if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "steps/s", "cores": int(os.environ.get("OMP_NUM_THREADS", cores)),
-            "kind": "port", "sample": f"{workload}: {steps} FP64 oracle steps after 1 warm-up step ({dt:.1f} s)"}
+    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{workload}: {steps} FP64 oracle steps ({iters} ADMM iterations) from the GPU's state at the "
+                      f"start of its timed region, {dt:.1f} s"}
 
 
 def main() -> int:
@@ -63,11 +99,14 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(batched scenes and a single scene both run on one device here)", file=sys.stderr)
+              "(the scene runs on one device; multi-GPU sharding is not built)", file=sys.stderr)
         return 2
+    if args.copy_calibration:
+        return copy_calibration()
 
     import parallel_admm_amd as pa
     from parallel_admm_amd.configs import SCENE_KIND, config_for
@@ -81,11 +120,12 @@ def main() -> int:
     for _ in range(args.warmup):
         backend.step()
     print(f"[bench] {args.workload}: warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    state = (*backend.get_state(), *backend.get_duals())
     backend.set_profiling(1)
     backend.kernel_time(reset=True)
     stats = []
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for _ in range(args.steps):
         stats.append(backend.step())
     t1 = time.perf_counter()
     spmv_ms, spmv_launches = backend.kernel_time()
@@ -106,16 +146,23 @@ def main() -> int:
           + ", ".join(f"{v:.2f}" for v in ph) + f"; iterations {st['iterations']}, PCG {st['global_iters']}",
           file=sys.stderr, flush=True)
 
-    # roofline of the PCG SpMV (the kernel moving the most bytes per launch in the global solve)
+    # roofline of the PCG SpMV (the global solve's kernel moving the most bytes per launch)
     sizes = backend.sizes()
     n = sizes["n_dyn"]
     nnz_off = sizes["nnz_a_off"]
     bytes_launch = spmv_bytes(n, nnz_off)
-    achieved = bytes_launch / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
+    # Kernel duration: HIP events on the backend's stream around 200
+    # back-to-back launches on the state the timed region left.  Every SpMV
+    # launch inside the timed region is also bracketed by its own event pair
+    # (event_bracket_ms), which adds the launch's dispatch latency.
+    kernel_ms = backend.spmv_time(200)
+    achieved = bytes_launch / (kernel_ms * 1e-3) / 1e9 if kernel_ms > 0 else 0.0
+    traffic, traffic_src = measured_traffic(args.workload)
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.workload, scene, cfg, args.cpu_seconds)
+        print(f"[bench] CPU oracle baseline (~{args.cpu_seconds:.0f} s) ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.workload, scene, cfg, state, args.cpu_seconds)
 
     out = {
         "metric": "sim steps/s at ADMM residual <=1e-4, plain vs learned warm-start; % HBM peak",
@@ -137,7 +184,7 @@ def main() -> int:
             "constraint_rows": scene.info["m_rows"],
             "residual_tol": args.tol,
             "admm_max_iter": cfg.admm_max_iter,
-            "warm_start": "plain (cold-start duals per detection)",
+            "warm_start": "plain (cold-start contact duals per detection)",
             "admm_iters_mean": float(np.mean(iters)),
             "converged_steps": conv,
             "contacts_last": stats[-1]["n_contacts"],
@@ -146,14 +193,16 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_cg_dir_spmv",
+            "kernel": ROOFLINE_KERNEL,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "bytes_per_launch": bytes_launch,
-            "avg_launch_ms": spmv_ms,
+            "avg_launch_ms": kernel_ms,
+            "event_bracket_ms": spmv_ms,
             "launches": spmv_launches,
         },
         "cpu_baseline": cpu,

@@ -274,9 +274,13 @@ pa_status pa_backend_phase_times(pa_backend* b, double* ms6);
 /* Sizes: [n_all, n_dyn, m_rows, nnz of the off-diagonal system matrix,
  * nnz of the damping matrix, nnz of the D^T gather, contacts of the last detection]. */
 pa_status pa_backend_get_sizes(pa_backend* b, int64_t* out7);
-/* Per-launch average duration (ms, HIP events on the backend's stream) and
- * launch count of the PCG SpMV kernel over the timed steps since the last call
- * with reset = 1. */
+/* which = 0: per-launch average (ms) of HIP event pairs around every PCG
+ * SpMV launch on the backend's stream during the steps since the last call
+ * with reset = 1 (each pair also holds the launch's dispatch latency), and the
+ * launch count.
+ * which = 1: launches the PCG SpMV kernel *launches (in) times back-to-back
+ * on the current device state between two events and returns the average per
+ * launch (ms); call between steps (it writes only PCG scratch). */
 pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, double* avg_ms, int64_t* launches);
 
 /* Message of the last failed call on this host thread (any handle). */

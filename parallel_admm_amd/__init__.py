@@ -468,6 +468,14 @@ class ADMMBackendHIP:
         self._err(lib().pa_backend_kernel_time(self._h, 0, int(reset), C.byref(avg), C.byref(n)), "kernel_time")
         return avg.value, n.value
 
+    def spmv_time(self, reps: int = 200) -> float:
+        """Per-launch time (ms, HIP events) of `reps` back-to-back launches of the
+        PCG SpMV kernel on the current device state (call between steps)."""
+        avg = C.c_double()
+        n = C.c_int64(int(reps))
+        self._err(lib().pa_backend_kernel_time(self._h, 1, 0, C.byref(avg), C.byref(n)), "spmv_time")
+        return avg.value
+
 
 def gpu_available() -> bool:
     """True when a HIP device is visible to the library (no allocation)."""

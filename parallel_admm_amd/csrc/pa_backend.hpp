@@ -132,6 +132,7 @@ struct Backend {
     void contact_project(bool first_iter);
     void contact_sweep_once();
     void rhs(bool want_residual, bool first_iter);
+    double spmv_time_ms(int reps);
     int global_solve(int max_iter, double tol);
     // cg_z = M^-1 r, partial r.z into cg_part[*, 6..8]; with skip_pre0 the
     // level-0 pre-smoothing was already done by k_cg_update_pre.

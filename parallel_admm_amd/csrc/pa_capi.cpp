@@ -518,8 +518,13 @@ pa_status pa_backend_phase_times(pa_backend* b, double* ms6) {
 
 pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, double* avg_ms, int64_t* launches) {
     BK({
-        PA_CHECK(which == 0, PA_ERR_INVALID, "unknown kernel id");
+        PA_CHECK(which == 0 || which == 1, PA_ERR_INVALID, "unknown kernel id");
         pa::Backend& B = *b->b;
+        if (which == 1) { // back-to-back SpMV launches: `launches` in = repetitions
+            PA_CHECK(avg_ms && launches, PA_ERR_INVALID, "null argument");
+            *avg_ms = B.spmv_time_ms((int)*launches);
+            return;
+        }
         if (avg_ms) *avg_ms = B.spmv_launches ? B.spmv_ms_sum / (double)B.spmv_launches : 0.0;
         if (launches) *launches = B.spmv_launches;
         if (reset) {

@@ -9,7 +9,8 @@
 //                   k_vertex_p     p = (x - x0 + Uc)/h + Gamma r_c      (contact on)
 //                   G x (k_contact_du, k_vertex_gather)                  (friction-cone sweeps)
 //                   k_rhs          b = M x~ + Damp x0 + h^2 D^T We^2 (z - Ue) + h^2 Wc (h p + x0 - Uc)
-//                   PCG            A x = b, Jacobi preconditioner, FP64 dot products
+//                   global solve   A x = b: FP64 residual + FP32 PCG correction (smoothed-aggregation
+//                                  AMG V-cycle or Jacobi preconditioner), FP64 dot products
 //                   k_dual_elem / k_dual_vert   Ue += D x - z, Uc += x - x0 - h p (+ residual sums)
 //   epilogue        k_end_step     v = (x - x0)/h
 // D is applied matrix-free from the constraint rest data; D^T We^2 is a
@@ -1577,6 +1578,47 @@ void Backend::rhs(bool want_res, bool) {
     hipLaunchKernelGGL(k_rhs, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)cfg.dt, cfg.enable_contact, want_res ? 1 : 0,
                        dt_ptr.p, dt_row.p, dt_coef.p, z.p, zprev.p, Ue.p, bconst.p, wch2.p, p.p, x0.p, Uc.p, b.p, res_part.p);
     PA_LAUNCH_CHECK();
+}
+
+// Per-launch time (ms) of the PCG SpMV kernel: `reps` launches back-to-back
+// on the backend's stream between two events, on the current device vectors
+// (run between steps: it only writes PCG scratch).  This is the kernel's own
+// duration plus the gap between consecutive launches; an event pair around a
+// single launch inside the solve also holds the dispatch latency.
+double Backend::spmv_time_ms(int reps) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "init first");
+    PA_CHECK(reps > 0 && reps <= 100000, PA_ERR_INVALID, "reps out of range");
+    const int nb = cg_blocks;
+    const bool fused = use_amg && amg.size() >= 2;
+    hipEvent_t e0, e1;
+    PA_HIP(hipEventCreate(&e0));
+    PA_HIP(hipEventCreate(&e1));
+    float ms = 0.f;
+    try {
+        PA_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < reps; r++) {
+            with_lpr(a_lpr, [&](auto c) {
+                constexpr int P = decltype(c)::value;
+                if (fused)
+                    hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, 0, 0, nb, cg_state.p, a_ptr.p, a_col.p,
+                                       a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)cg_p.p, cg_p2.p, cg_q.p, cg_part.p);
+                else
+                    hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
+                                       (const float*)cg_p.p, cg_q.p, cg_part.p);
+            });
+            PA_LAUNCH_CHECK();
+        }
+        PA_HIP(hipEventRecord(e1, stream));
+        PA_HIP(hipEventSynchronize(e1));
+        PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
 }
 
 hipEvent_t Backend::next_event() {

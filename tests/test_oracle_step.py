@@ -89,7 +89,7 @@ def test_pinned_vertices_hold(pa):
     assert np.abs(x[free, 1] - x0[free, 1]).max() > 1e-2
 
 
-def test_box_prefilter_equals_all_pairs(pa):
+def test_grid_broad_phase_equals_all_pairs(pa):
     import sys, os
     sys.path.insert(0, os.path.dirname(__file__))
     from helpers import folded_cloth
