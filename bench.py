@@ -64,15 +64,18 @@ def copy_calibration() -> int:
     return 0
 
 
-def cpu_baseline(workload: str, scene, cfg, state, seconds: float) -> dict:
+def cpu_baseline(workload: str, scene, cfg, state, tracks, frame: int, seconds: float) -> dict:
     """FP64 oracle (prefactored sparse LLT, OpenMP local step and broad phase)
     on the host cores, started from the GPU's state at the beginning of its
-    timed region and run for about `seconds` (at least one step)."""
+    timed region (obstacle tracks resumed at the same frame) and run for about
+    `seconds` (at least one step)."""
     from oracle.pyoracle import Oracle
 
     cores = os.cpu_count() or 1
     threads = int(os.environ.get("OMP_NUM_THREADS", cores))
     o = Oracle(scene.mesh(), cfg)
+    for first, count, pos, quat in tracks:
+        o.add_rigid_track(first, count, pos[frame:], quat[frame:])
     x, v, ue, uc = state
     o.set_state(x, v)
     o.set_duals(ue, uc)
@@ -109,13 +112,17 @@ def main() -> int:
         return copy_calibration()
 
     import parallel_admm_amd as pa
-    from parallel_admm_amd.configs import SCENE_KIND, config_for
+    from parallel_admm_amd.configs import SCENE_KIND, config_for, workload_tracks
 
     scene = pa.Scene.generate(SCENE_KIND[args.workload])
     cfg = config_for(args.workload)
     cfg.residual_tol = args.tol
     cfg.residual_interval = 1
     backend = pa.ADMMBackendHIP.from_scene(scene, cfg)
+    # moving obstacles of the workload, one frame per step, long enough for the whole run
+    tracks = workload_tracks(args.workload, scene, args.warmup + args.steps + 64)
+    for first, count, pos, quat in tracks:
+        backend.add_rigid_track(first, count, pos, quat, frames_per_step=1)
     backend.set_sync_to_host(False)
     for _ in range(args.warmup):
         backend.step()
@@ -162,7 +169,7 @@ def main() -> int:
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU oracle baseline (~{args.cpu_seconds:.0f} s) ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.workload, scene, cfg, state, args.cpu_seconds)
+        cpu = cpu_baseline(args.workload, scene, cfg, state, tracks, args.warmup, args.cpu_seconds)
 
     out = {
         "metric": "sim steps/s at ADMM residual <=1e-4, plain vs learned warm-start; % HBM peak",
@@ -185,6 +192,7 @@ def main() -> int:
             "residual_tol": args.tol,
             "admm_max_iter": cfg.admm_max_iter,
             "warm_start": "plain (cold-start contact duals per detection)",
+            "moving_obstacles": len(tracks),
             "admm_iters_mean": float(np.mean(iters)),
             "converged_steps": conv,
             "contacts_last": stats[-1]["n_contacts"],

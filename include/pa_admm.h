@@ -250,6 +250,20 @@ pa_status pa_backend_set_duals(pa_backend* b, const double* ue, const double* uc
  * src/solver/animator.cpp:199-205): positions and velocities of [n_dyn, n_all). */
 pa_status pa_backend_set_static_state(pa_backend* b, const double* x_static, const double* v_static);
 
+/* Rigid track for the static vertices [first, first + count) (≙ a rigid
+ * ScriptAnimator script attached to a static mesh, src/solver/animator.cpp:9-39,
+ * 83-130): frame k places them at pos[k] + q[k] (x) rest, rest = the positions
+ * given to pa_backend_init, q[k] = quat_xyzw[4k..4k+3] in the npz order
+ * (x, y, z, w) (normalised).  Registration places frame 0 with zero velocity;
+ * each step first advances the frame counter by frames_per_step (the reference
+ * advances it once in animate_all and once per rigid mesh in animate_rigid, so
+ * one rigid mesh moves 2 frames per step) and sets the static velocity to
+ * (x - x_prev) / h; past the last frame the vertices stay there with zero
+ * velocity.  Needs n_frames >= 3; tracks may not overlap; reset() re-places
+ * frame 0.  Call after pa_backend_init. */
+pa_status pa_backend_add_rigid_track(pa_backend* b, int32_t first, int32_t count, int32_t n_frames, const double* pos,
+                                     const double* quat_xyzw, int32_t frames_per_step);
+
 /* Convenience: runs the whole call order above for a scene (creates and
  * attaches its own detector when contact is enabled). */
 pa_status pa_backend_create_from_scene(const pa_scene* s, const pa_admm_config* cfg, const pa_device_opts* opts,

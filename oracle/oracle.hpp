@@ -1241,6 +1241,68 @@ template <class R> struct Solver {
     std::vector<R> r_c;
     std::vector<double> residual_log; // r_p, r_d per check
     bool brute_force = false;
+    // Rigid tracks of static vertex ranges (ScriptAnimator rigid scripts,
+    // src/solver/animator.cpp:9-39, 83-130; see add_rigid_track).
+    struct Track {
+        int first = 0, count = 0, fps = 1, cur = 0, frames = 0;
+        std::vector<double> trans, quat; // 3 and 4 (w, x, y, z) per frame
+        std::vector<double> rest;        // positions at set-up
+    };
+    std::vector<Track> tracks;
+
+    // x = t + q (x) rest (unit-quaternion rotation v + w T + u x T, T = 2 u x v);
+    // v_static = (x - x_prev) / h, or zero at registration and past the end.
+    void place_track(Track& t, int f, bool zero_v) {
+        const double* tr = &t.trans[3 * (size_t)f];
+        const double* q = &t.quat[4 * (size_t)f];
+        const double hinv = 1.0 / cfg.dt;
+        for (int i = 0; i < t.count; i++) {
+            const double rx = t.rest[3 * i], ry = t.rest[3 * i + 1], rz = t.rest[3 * i + 2];
+            const double Tx = 2.0 * (q[2] * rz - q[3] * ry), Ty = 2.0 * (q[3] * rx - q[1] * rz), Tz = 2.0 * (q[1] * ry - q[2] * rx);
+            const double nx = tr[0] + (rx + q[0] * Tx + (q[2] * Tz - q[3] * Ty));
+            const double ny = tr[1] + (ry + q[0] * Ty + (q[3] * Tx - q[1] * Tz));
+            const double nz = tr[2] + (rz + q[0] * Tz + (q[1] * Ty - q[2] * Tx));
+            const int g = t.first + i;
+            const double xn[3] = {nx, ny, nz};
+            for (int k = 0; k < 3; k++) {
+                v[3 * g + k] = zero_v ? R(0) : R((xn[k] - double(x[3 * g + k])) * hinv);
+                x[3 * g + k] = R(xn[k]);
+            }
+        }
+    }
+    // Frame 0 with zero velocity at registration (the initial animate_all,
+    // admm_impl_gpu.cu:143-144); each step advances fps frames first.
+    void add_rigid_track(int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps) {
+        if (first < n || count <= 0 || first + count > n_all || frames < 3 || fps < 1)
+            throw std::runtime_error("bad rigid track");
+        Track t;
+        t.first = first;
+        t.count = count;
+        t.frames = frames;
+        t.fps = fps;
+        t.trans.assign(pos, pos + 3 * (size_t)frames);
+        t.quat.resize(4 * (size_t)frames);
+        for (int f = 0; f < frames; f++) {
+            const double* q = quat_xyzw + 4 * (size_t)f;
+            const double nq = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+            t.quat[4 * f] = q[3] / nq;
+            t.quat[4 * f + 1] = q[0] / nq;
+            t.quat[4 * f + 2] = q[1] / nq;
+            t.quat[4 * f + 3] = q[2] / nq;
+        }
+        t.rest.assign(S.x.begin() + 3 * (size_t)first, S.x.begin() + 3 * (size_t)(first + count));
+        t.cur = 1;
+        place_track(t, 0, true);
+        tracks.push_back(std::move(t));
+    }
+    void animate_tracks() {
+        for (auto& t : tracks) {
+            t.cur += t.fps;
+            const bool end = t.cur >= t.frames;
+            if (end) t.cur = t.frames;
+            place_track(t, t.cur - 1, end);
+        }
+    }
 
     void setup(const SceneInput& in, const Config& c) {
         S = in;
@@ -1470,6 +1532,7 @@ template <class R> struct Solver {
         const double hd = cfg.dt;
         const R eps = R(1.0 / (hd * hd * cfg.kappa + hd * cfg.beta));
         const R gam = R(hd * cfg.kappa / (hd * cfg.kappa + cfg.beta));
+        animate_tracks();
         std::vector<R> x0 = x, v0 = v;
         for (int i = 0; i < n; i++) {
             if (pinned[i]) {

@@ -135,6 +135,17 @@ int orc_step(void* hp, orc_stats* out) {
     return 0;
 }
 
+// Rigid track of static vertices (see Solver::add_rigid_track); 0 on success, 1 on bad input.
+int orc_add_rigid_track(void* hp, int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps) {
+    auto* h = static_cast<Handle*>(hp);
+    try {
+        DISPATCH(h, s.add_rigid_track(first, count, frames, pos, quat_xyzw, fps));
+    } catch (const std::exception&) {
+        return 1;
+    }
+    return 0;
+}
+
 // sizes: n_all, n_dyn, m, nnz_D, nnz_A, n_surf_v, n_surf_t, n_surf_e, n_constraints, n_contacts, nnz_L
 void orc_sizes(void* hp, long long* out) {
     auto* h = static_cast<Handle*>(hp);

@@ -123,6 +123,12 @@ class Oracle:
         b = None if uc is None else np.ascontiguousarray(uc, np.float64).reshape(-1)
         lib().orc_set_duals(self._h, _p(a), _p(b))
 
+    def add_rigid_track(self, first, count, pos, quat_xyzw, frames_per_step=1):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(quat_xyzw, np.float64).reshape(-1, 4)
+        if lib().orc_add_rigid_track(self._h, int(first), int(count), len(pos), _p(pos), _p(q), int(frames_per_step)) != 0:
+            raise ValueError("bad rigid track")
+
     def mass(self):
         m = np.zeros(self.n_all)
         lib().orc_get_mass(self._h, _p(m))

@@ -37,7 +37,7 @@ EXPORTS = [
     "pa_backend_reset", "pa_backend_get_state", "pa_backend_set_state", "pa_backend_get_duals",
     "pa_backend_set_duals", "pa_backend_set_static_state", "pa_backend_create_from_scene",
     "pa_backend_project_elastic", "pa_backend_project_contacts", "pa_backend_global_solve",
-    "pa_backend_set_profiling", "pa_backend_get_sizes", "pa_backend_phase_times", "pa_backend_kernel_time", "pa_last_error_message",
+    "pa_backend_add_rigid_track", "pa_backend_set_profiling", "pa_backend_get_sizes", "pa_backend_phase_times", "pa_backend_kernel_time", "pa_last_error_message",
 ]
 
 
@@ -467,6 +467,18 @@ class ADMMBackendHIP:
         n = C.c_int64()
         self._err(lib().pa_backend_kernel_time(self._h, 0, int(reset), C.byref(avg), C.byref(n)), "kernel_time")
         return avg.value, n.value
+
+    def add_rigid_track(self, first: int, count: int, pos, quat_xyzw, frames_per_step: int = 1) -> None:
+        """Rigid animation of the static vertices [first, first + count): frame k
+        places them at pos[k] + q[k] (x) rest (quaternions in (x, y, z, w) order);
+        each step advances `frames_per_step` frames (the reference's animator
+        moves a single rigid mesh 2 frames per step)."""
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+        q = np.ascontiguousarray(quat_xyzw, np.float64).reshape(-1, 4)
+        if len(pos) != len(q):
+            raise ValueError("pos and quat_xyzw need the same number of frames")
+        self._err(lib().pa_backend_add_rigid_track(self._h, int(first), int(count), len(pos), _ptr(pos), _ptr(q),
+                                                   int(frames_per_step)), "add_rigid_track")
 
     def spmv_time(self, reps: int = 200) -> float:
         """Per-launch time (ms, HIP events) of `reps` back-to-back launches of the

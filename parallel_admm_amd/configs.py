@@ -5,9 +5,12 @@ choice (the reference scenes use 20..2000).
 
   cloth32   32x32 cloth over an icosphere: FP64 CPU-runnable case
   tetbox    20^3-cell 5-tet box dropped on a ground quad
-  cloth256  256x256 self-contact cloth over a sphere and a cylinder
+  cloth256  256x256 self-contact cloth over a sphere and a cylinder that
+            rotates about its axis at 1 rad/s (rigid track, workload_tracks)
 """
 from __future__ import annotations
+
+import numpy as np
 
 from . import AdmmConfig
 
@@ -32,3 +35,33 @@ def config_for(name: str) -> AdmmConfig:
                           kappa=1e6, beta=1e5, thickness=1e-3, damp_kl=1e-4, damp_km=0.0, w_scale=50.0,
                           max_contacts=1 << 16)
     raise KeyError(name)
+
+
+def rotation_track(center, axis, omega: float, dt: float, frames: int):
+    """Rigid track of a rotation about `axis` through `center` at `omega` rad/s,
+    one frame per step of `dt`: frame k has angle omega k dt, quaternion
+    (sin(a/2) axis, cos(a/2)) in (x, y, z, w) order and translation
+    center - R center, so x = pos + R rest rotates rest about the centre."""
+    c = np.asarray(center, np.float64)
+    u = np.asarray(axis, np.float64)
+    u = u / np.linalg.norm(u)
+    ang = omega * dt * np.arange(frames)
+    quat = np.concatenate([np.sin(ang / 2)[:, None] * u[None, :], np.cos(ang / 2)[:, None]], axis=1)
+    pos = np.empty((frames, 3))
+    for k, a in enumerate(ang):  # Rodrigues
+        K = np.array([[0, -u[2], u[1]], [u[2], 0, -u[0]], [-u[1], u[0], 0]])
+        Rk = np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * (K @ K)
+        pos[k] = c - Rk @ c
+    return pos, quat
+
+
+def workload_tracks(name: str, scene, frames: int) -> list:
+    """Rigid tracks of a workload as (first, count, pos, quat_xyzw) with one
+    frame per step.  cloth256: the cylinder (the last 48 x 41 vertices, axis z
+    through (0.22, 0.05)) turns at 1 rad/s."""
+    if name != "cloth256":
+        return []
+    n_all = scene.info["n_all"]
+    count = 48 * 41
+    pos, quat = rotation_track((0.22, 0.05, 0.0), (0.0, 0.0, 1.0), 1.0, config_for(name).dt, frames)
+    return [(n_all - count, count, pos, quat)]

@@ -88,6 +88,14 @@ struct Backend {
     int amg_nc = 0, amg_ncp = 0; // coarsest size and its row stride (multiple of 4)
     bool use_amg = false;
     std::vector<int> amg_sizes;
+    // rigid animation of static vertex ranges (ScriptAnimator rigid tracks)
+    struct RigidTrack {
+        int first = 0, count = 0, fps = 1, cur = 0, frames = 0;
+        std::vector<double> trans, quat; // 3 and 4 (w, x, y, z) per frame
+        DBuf<double> rest, prev;         // 3 count each: rest positions, last placed positions
+    };
+    std::vector<std::unique_ptr<RigidTrack>> tracks;
+    std::vector<double> x_init_h; // positions given to init (rest positions of the tracks)
     // contacts
     Detector* det = nullptr;
     std::unique_ptr<Detector> own_det;
@@ -133,6 +141,10 @@ struct Backend {
     void contact_sweep_once();
     void rhs(bool want_residual, bool first_iter);
     double spmv_time_ms(int reps);
+    // Rigid track for static vertices [first, first + count): frame k places
+    // them at trans_k + R(q_k) rest; see pa_backend_add_rigid_track.
+    void add_rigid_track(int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps);
+    void animate_tracks(bool place_first_frame);
     int global_solve(int max_iter, double tol);
     // cg_z = M^-1 r, partial r.z into cg_part[*, 6..8]; with skip_pre0 the
     // level-0 pre-smoothing was already done by k_cg_update_pre.

@@ -413,6 +413,11 @@ pa_status pa_backend_set_duals(pa_backend* b, const double* ue, const double* uc
 pa_status pa_backend_set_static_state(pa_backend* b, const double* xs, const double* vs) { BK(b->b->set_static_state(xs, vs)); }
 pa_status pa_backend_set_profiling(pa_backend* b, int32_t mode) { BK(b->b->profiling = mode & 3); }
 
+pa_status pa_backend_add_rigid_track(pa_backend* b, int32_t first, int32_t count, int32_t n_frames, const double* pos,
+                                     const double* quat_xyzw, int32_t frames_per_step) {
+    BK(b->b->add_rigid_track(first, count, n_frames, pos, quat_xyzw, frames_per_step));
+}
+
 pa_status pa_backend_create_from_scene(const pa_scene* sc, const pa_admm_config* cfg, const pa_device_opts* o, pa_backend** out) {
     return guard(nullptr, [&] {
         PA_CHECK(sc && cfg && out, PA_ERR_INVALID, "null argument");

@@ -1038,6 +1038,32 @@ __global__ void __launch_bounds__(BLK) k_amg_coarse(int nc, int ncp, const float
     }
 }
 
+// Rigid animation of static vertices (reference ScriptAnimator::animate_rigid,
+// src/solver/animator.cpp:83-130): x = t + q (x) rest with the unit quaternion
+// rotation v' = v + w T + u x T, T = 2 (u x v); v_static = (x - x_prev) / h,
+// or 0 once the track has ended.  FP64, positions stored to FP32.
+__global__ void k_animate_rigid(int first, int count, const double* __restrict__ rest, double* __restrict__ prev, double tx,
+                                double ty, double tz, double qw, double qx, double qy, double qz, double hinv, int zero_v,
+                                float* __restrict__ x, float* __restrict__ v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double rx = rest[3 * i], ry = rest[3 * i + 1], rz = rest[3 * i + 2];
+    const double Tx = 2.0 * (qy * rz - qz * ry), Ty = 2.0 * (qz * rx - qx * rz), Tz = 2.0 * (qx * ry - qy * rx);
+    const double nx = tx + (rx + qw * Tx + (qy * Tz - qz * Ty));
+    const double ny = ty + (ry + qw * Ty + (qz * Tx - qx * Tz));
+    const double nz = tz + (rz + qw * Tz + (qx * Ty - qy * Tx));
+    const int g = first + i;
+    x[3 * g] = (float)nx;
+    x[3 * g + 1] = (float)ny;
+    x[3 * g + 2] = (float)nz;
+    v[3 * g] = zero_v ? 0.f : (float)((nx - prev[3 * i]) * hinv);
+    v[3 * g + 1] = zero_v ? 0.f : (float)((ny - prev[3 * i + 1]) * hinv);
+    v[3 * g + 2] = zero_v ? 0.f : (float)((nz - prev[3 * i + 2]) * hinv);
+    prev[3 * i] = nx;
+    prev[3 * i + 1] = ny;
+    prev[3 * i + 2] = nz;
+}
+
 } // namespace
 
 // ===========================================================================
@@ -1162,7 +1188,69 @@ void Backend::init(const double* x0h, const double* massh, int n_all_, int n_dyn
         pinned_h[pins[i]] = 1;
     }
     assemble_and_upload(x0h);
+    x_init_h.assign(x0h, x0h + 3 * (size_t)n_all);
     initialized = true;
+}
+
+// Registers a rigid track and places its vertices at frame 0 with zero
+// velocity, as the reference's initial animate_all does (admm_impl_gpu.cu:143-144).
+void Backend::add_rigid_track(int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "add_rigid_track before init");
+    PA_CHECK(pos && quat_xyzw, PA_ERR_INVALID, "null track data");
+    PA_CHECK(first >= n && count > 0 && (int64_t)first + count <= n_all, PA_ERR_INVALID, "track range must lie within the static vertices");
+    PA_CHECK(frames >= 3, PA_ERR_INVALID, "track needs at least 3 frames");
+    PA_CHECK(fps >= 1, PA_ERR_INVALID, "frames per step must be >= 1");
+    for (const auto& t : tracks)
+        PA_CHECK(first + count <= t->first || t->first + t->count <= first, PA_ERR_INVALID, "tracks overlap");
+    auto T = std::make_unique<RigidTrack>();
+    T->first = first;
+    T->count = count;
+    T->frames = frames;
+    T->fps = fps;
+    T->trans.assign(pos, pos + 3 * (size_t)frames);
+    T->quat.resize(4 * (size_t)frames);
+    for (int f = 0; f < frames; f++) { // npz order (x, y, z, w) -> (w, x, y, z), animator.cpp:32
+        const double* q = quat_xyzw + 4 * (size_t)f;
+        const double nq = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+        PA_CHECK(nq > 0.5 && nq < 2.0, PA_ERR_INVALID, "track quaternion far from unit length");
+        T->quat[4 * f] = q[3] / nq;
+        T->quat[4 * f + 1] = q[0] / nq;
+        T->quat[4 * f + 2] = q[1] / nq;
+        T->quat[4 * f + 3] = q[2] / nq;
+    }
+    T->rest.alloc(3 * (size_t)count);
+    T->rest.upload(x_init_h.data() + 3 * (size_t)first, 3 * (size_t)count, stream);
+    T->prev.alloc(3 * (size_t)count);
+    tracks.push_back(std::move(T));
+    animate_tracks(true);
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+// Per step (before the prologue): frame counter += frames per step; past the
+// end the vertices stay at the last frame with zero velocity
+// (animator.cpp:89-103).  With place_first_frame every track is put at frame 0.
+void Backend::animate_tracks(bool place_first_frame) {
+    const double hinv = 1.0 / cfg.dt;
+    for (auto& tp : tracks) {
+        RigidTrack& T = *tp;
+        int f;
+        bool zero_v;
+        if (place_first_frame) {
+            T.cur = 1;
+            f = 0;
+            zero_v = true;
+        } else {
+            T.cur += T.fps;
+            zero_v = T.cur >= T.frames;
+            if (zero_v) T.cur = T.frames;
+            f = T.cur - 1;
+        }
+        const double* t = T.trans.data() + 3 * (size_t)f;
+        const double* q = T.quat.data() + 4 * (size_t)f;
+        hipLaunchKernelGGL(k_animate_rigid, dim3(grid_for(T.count, BLK)), dim3(BLK), 0, stream, T.first, T.count, T.rest.p, T.prev.p,
+                           t[0], t[1], t[2], q[0], q[1], q[2], q[3], hinv, zero_v ? 1 : 0, x.p, v.p);
+        PA_LAUNCH_CHECK();
+    }
 }
 
 void Backend::assemble_and_upload(const double* x0h) {
@@ -1870,6 +1958,7 @@ void Backend::step_impl(pa_step_stats* st) {
     PA_CHECK(!contact || det, PA_ERR_INVALID, "contact enabled but no detector attached");
     auto t0 = std::chrono::steady_clock::now();
     pa_step_stats S{};
+    if (!tracks.empty()) animate_tracks(false);
     const double hd = cfg.dt;
     const float eps = (float)(1.0 / (hd * hd * cfg.kappa + hd * cfg.beta));
     const float gam_ = (float)(hd * cfg.kappa / (hd * cfg.kappa + cfg.beta));
@@ -1964,6 +2053,10 @@ void Backend::reset(const double* x0h) {
     PA_HIP(hipStreamSynchronize(stream));
     x_host.assign(x0h, x0h + 3 * (size_t)n_all);
     std::fill(v_host.begin(), v_host.end(), 0.0);
+    if (!tracks.empty()) {
+        animate_tracks(true);
+        PA_HIP(hipStreamSynchronize(stream));
+    }
 }
 
 void Backend::get_state(double* xo, double* vo) {

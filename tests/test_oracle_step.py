@@ -158,3 +158,44 @@ def test_single_contact_projection_by_hand(pa):
         np.testing.assert_allclose(rc[0], du, atol=1e-12, err_msg=case)
         np.testing.assert_allclose(pp[0], pv + G * du, atol=1e-12, err_msg=case)
         np.testing.assert_allclose(pp[1:3], 0.0, atol=0)
+
+
+def test_rigid_track_follows_rotation(pa):
+    """Rigid track (ScriptAnimator rigid script semantics): after k steps the
+    obstacle is the rest mesh rotated by omega k dt about its axis (Rodrigues,
+    independent of the quaternion path), its velocity is the backward
+    difference, and past the last frame it stays with zero velocity; with 2
+    frames per step it advances two frames."""
+    from parallel_admm_amd.configs import config_for, rotation_track
+
+    cfg = config_for("cloth32")
+    cfg.enable_contact = False
+    sc = pa.Scene.generate(4, side=8, radius=0.2, drop=0.2)
+    m = sc.mesh()
+    n, n_all = sc.info["n_dyn"], sc.info["n_all"]
+    rest = m["x"][n:]
+    omega, frames = 2.0, 6
+    pos, quat = rotation_track((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), omega, cfg.dt, frames)
+
+    def rot(a):
+        c, s = np.cos(a), np.sin(a)
+        return np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])  # about +y
+
+    for fps in (1, 2):
+        o = Oracle(m, cfg)
+        o.add_rigid_track(n, n_all - n, pos, quat, frames_per_step=fps)
+        x, v = o.state()
+        np.testing.assert_allclose(x[n:], rest, atol=1e-14)
+        np.testing.assert_array_equal(v[n:], 0.0)
+        prev = x[n:].copy()
+        for k in range(1, 5):
+            o.step()
+            x, v = o.state()
+            f = min(1 + k * fps, frames) - 1
+            ended = 1 + k * fps >= frames
+            np.testing.assert_allclose(x[n:], rest @ rot(omega * cfg.dt * f).T, atol=1e-13)
+            want_v = 0.0 if ended else (x[n:] - prev) / cfg.dt
+            np.testing.assert_allclose(v[n:], want_v, atol=1e-10)
+            prev = x[n:].copy()
+    with pytest.raises(ValueError):
+        Oracle(m, cfg).add_rigid_track(0, 5, pos, quat)  # dynamic vertices cannot be animated
