@@ -1,0 +1,471 @@
+"""Scene files, frame export and run comparison in the reference's formats.
+
+- read_obj / read_msh: triangle meshes (OBJ) and linear tet meshes (gmsh
+  .msh 2.2 ASCII or binary, 4.1 ASCII), as MeshData::process_triangle_mesh /
+  process_tetrahedral_mesh read them (src/mesh/mesh_container.h:492-596,
+  src/mutils/mesh_io.h:15-80).
+- load_scene: a scene TOML in the APPConfig schema
+  (src/config/app_config.h:158-358) -> Scene + AdmmConfig + rigid tracks, with
+  load_meshes_from_config's order and transforms
+  (src/config/mesh_from_config.h:18-85) and build_solver_config's key mapping
+  (src/solver/solver_config_applier.cpp:5-53).
+- export_frame / write_step_metrics_csv: OBJ/PLY frames named
+  <scene>_<NNNNN> and step_metrics.csv (src/mesh/export_frame.h:14-54,
+  src/solver/solver.h:38-45).
+- compare_runs: frames.csv with the columns of tools/compare_solvers.py:143-198
+  plus the relative trajectory error.
+- run_scene: the headless frame loop (src/main.cpp:29-94 without the GUI).
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import math
+import os
+import re
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PI_REFERENCE = 3.1415926  # the constant load_meshes_from_config converts degrees with (mesh_from_config.h:24)
+STATIC_DENSITY = 1e12     # static meshes (mesh_from_config.h:67)
+
+
+# ---------------------------------------------------------------------------
+# Mesh readers
+# ---------------------------------------------------------------------------
+def read_obj(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Vertices (n x 3) and 0-based triangles (m x 3) of an OBJ file; polygon
+    faces are fanned into triangles, texture/normal indices are ignored."""
+    verts, faces = [], []
+    with open(path, encoding="utf-8", errors="replace") as f:
+        for line in f:
+            tok = line.split()
+            if not tok:
+                continue
+            if tok[0] == "v":
+                verts.append([float(t) for t in tok[1:4]])
+            elif tok[0] == "f":
+                idx = [int(t.split("/")[0]) for t in tok[1:]]
+                idx = [i - 1 if i > 0 else len(verts) + i for i in idx]
+                for k in range(1, len(idx) - 1):
+                    faces.append([idx[0], idx[k], idx[k + 1]])
+    return np.array(verts, np.float64).reshape(-1, 3), np.array(faces, np.int32).reshape(-1, 3)
+
+
+def _msh_sections(text: str) -> dict:
+    out, name, buf = {}, None, []
+    for line in text.splitlines():
+        s = line.strip()
+        if s.startswith("$End"):
+            out[name] = buf
+            name, buf = None, []
+        elif s.startswith("$"):
+            name, buf = s[1:], []
+        elif name is not None and s:
+            buf.append(s)
+    return out
+
+
+# nodes per element for the gmsh element types a volume mesh file may hold
+_MSH_NODES = {1: 2, 2: 3, 3: 4, 4: 4, 5: 8, 6: 6, 7: 5, 8: 3, 9: 6, 10: 9, 11: 10, 15: 1}
+
+
+def _read_msh22_binary(raw: bytes, path: str) -> tuple[np.ndarray, np.ndarray]:
+    """gmsh 2.2 binary: '$Nodes\n<N>\n' then N records (int32 id, 3 float64),
+    '$Elements\n<M>\n' then blocks (int32 type, count, ntags; per element
+    int32 id, tags, nodes)."""
+    i = raw.index(b"$Nodes\n") + len(b"$Nodes\n")
+    j = raw.index(b"\n", i)
+    nn = int(raw[i:j])
+    rec = np.frombuffer(raw, dtype=[("id", "<i4"), ("x", "<f8", (3,))], count=nn, offset=j + 1)
+    nodes = rec["x"].astype(np.float64)
+    i = raw.index(b"$Elements\n", j + 1 + 28 * nn) + len(b"$Elements\n")
+    j = raw.index(b"\n", i)
+    ne = int(raw[i:j])
+    off, seen, tets = j + 1, 0, []
+    while seen < ne:
+        etype, cnt, ntags = np.frombuffer(raw, "<i4", 3, off)
+        off += 12
+        if int(etype) not in _MSH_NODES:
+            raise ValueError(f"{path}: unsupported element type {int(etype)}")
+        width = 1 + int(ntags) + _MSH_NODES[int(etype)]
+        blk = np.frombuffer(raw, "<i4", int(cnt) * width, off).reshape(int(cnt), width)
+        off += 4 * int(cnt) * width
+        if int(etype) == 4:
+            tets.append(blk[:, 1 + int(ntags):] - 1)
+        seen += int(cnt)
+    t = np.concatenate(tets).astype(np.int32) if tets else np.zeros((0, 4), np.int32)
+    return nodes, t
+
+
+def read_msh(path: str) -> tuple[np.ndarray, np.ndarray]:
+    """Nodes (n x 3) and 0-based linear tets (element type 4) of a gmsh .msh
+    file: format 2.2 (ASCII or binary) or 4.1 (ASCII).  Nodes are taken in file
+    order and element node tags minus one are the indices, as the reference's
+    reader does (mesh_io.h:42-60)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    head = raw[: raw.index(b"$EndMeshFormat")].decode("ascii", "replace").split("\n")
+    fmt = head[1].split() if len(head) > 1 else []
+    if len(fmt) >= 2 and fmt[1] == "1":
+        if not fmt[0].startswith("2"):
+            raise ValueError(f"{path}: binary .msh files are supported in format 2.2 only")
+        return _read_msh22_binary(raw, path)
+    text = raw.decode("ascii")
+    sec = _msh_sections(text)
+    if "MeshFormat" not in sec:
+        raise ValueError(f"{path}: no $MeshFormat section")
+    version = sec["MeshFormat"][0].split()[0]
+    nodes, tets = [], []
+    if version.startswith("2"):
+        lines = sec["Nodes"]
+        for ln in lines[1:1 + int(lines[0])]:
+            nodes.append([float(t) for t in ln.split()[1:4]])
+        lines = sec["Elements"]
+        for ln in lines[1:1 + int(lines[0])]:
+            t = [int(v) for v in ln.split()]
+            if t[1] == 4:
+                tets.append([v - 1 for v in t[3 + t[2]:3 + t[2] + 4]])
+    elif version.startswith("4"):
+        lines = sec["Nodes"]
+        nblocks = int(lines[0].split()[0])
+        i = 1
+        for _ in range(nblocks):
+            nin = int(lines[i].split()[3])
+            i += 1 + nin  # block header, then node tags
+            for k in range(nin):
+                nodes.append([float(v) for v in lines[i + k].split()[:3]])
+            i += nin
+        lines = sec["Elements"]
+        nblocks = int(lines[0].split()[0])
+        i = 1
+        for _ in range(nblocks):
+            dim, _tag, etype, nin = (int(v) for v in lines[i].split()[:4])
+            i += 1
+            if dim == 3 and etype == 4:
+                for k in range(nin):
+                    t = [int(v) for v in lines[i + k].split()]
+                    tets.append([v - 1 for v in t[1:5]])
+            i += nin
+    else:
+        raise ValueError(f"{path}: unsupported .msh version {version}")
+    return np.array(nodes, np.float64).reshape(-1, 3), np.array(tets, np.int32).reshape(-1, 4)
+
+
+def boundary_facets(tets: np.ndarray) -> np.ndarray:
+    """Faces that belong to exactly one tet, oriented outward for positively
+    oriented tets, in tet order then local face order (each tet's faces
+    (1,2,3), (0,3,2), (0,1,3), (0,2,1))."""
+    local = ((1, 2, 3), (0, 3, 2), (0, 1, 3), (0, 2, 1))
+    faces = [tuple(int(t[a]) for a in loc) for t in tets for loc in local]
+    count: dict = {}
+    for f in faces:
+        key = tuple(sorted(f))
+        count[key] = count.get(key, 0) + 1
+    out = [f for f in faces if count[tuple(sorted(f))] == 1]
+    return np.array(out, np.int32).reshape(-1, 3)
+
+
+# ---------------------------------------------------------------------------
+# Scene TOML
+# ---------------------------------------------------------------------------
+def euler_xyz_degrees(rx: float, ry: float, rz: float) -> np.ndarray:
+    """R = Rz Ry Rx for angles in degrees (q_z * q_y * q_x of
+    mesh_from_config.h:25-30, with its pi)."""
+    ax, ay, az = (PI_REFERENCE * a / 180.0 for a in (rx, ry, rz))
+    cx, sx, cy, sy, cz, sz = math.cos(ax), math.sin(ax), math.cos(ay), math.sin(ay), math.cos(az), math.sin(az)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+@dataclass
+class SceneSpec:
+    """What load_scene returns: the scene, its solver settings, the rigid
+    tracks of animated static meshes and the frame-loop settings."""
+    scene: object
+    config: object
+    tracks: list = field(default_factory=list)   # (first, count, pos, quat_xyzw, frames_per_step)
+    scene_name: str = ""
+    end_frame: int = 2000
+    sub_step: int = 1
+    mesh_ranges: list = field(default_factory=list)  # (first vertex, count, type) per mesh in vertex order
+
+
+def _get(d: dict, dotted: str, default=None):
+    cur = d
+    for k in dotted.split("."):
+        if not isinstance(cur, dict) or k not in cur:
+            return default
+        cur = cur[k]
+    return cur
+
+
+def load_scene(toml_path: str, resource_dir: str) -> SceneSpec:
+    """Builds a scene from a TOML file in the reference's schema.
+
+    Dynamic 'tri' and 'tet' meshes come first in file order, then 'static'
+    meshes (density 1e12); each mesh is centred when listed in
+    mesh_post.center_ids, then scaled, rotated (R = Rz Ry Rx in degrees) and
+    translated.  Per-mesh k.stretch / k.bending / density give the triangle,
+    bending and tet stiffness and the lumped mass; global.pin lists pinned
+    vertices (static ones are skipped, mesh_to_constraint.h:43-44).  A static
+    mesh with an `animate` .npz (arrays 'pos' (F x 3) and 'quat' (F x 4,
+    x y z w)) becomes a rigid track; the reference's animator advances one
+    shared frame counter once per step and once more per rigid mesh
+    (animator.cpp:83-86, 199-205), so with K rigid meshes track j starts at
+    frame j and moves K + 1 frames per step.  Point-sequence animations are
+    not supported."""
+    try:
+        import tomllib as toml_mod  # Python >= 3.11
+    except ImportError:
+        import tomli as toml_mod
+    from . import AdmmConfig, Scene
+
+    with open(toml_path, "rb") as f:
+        cfg = toml_mod.load(f)
+    meshes = cfg.get("mesh", [])
+    center_ids = set(_get(cfg, "mesh_post.center_ids", []) or [])
+    xs, dens, tris, tri_k, bend_k, tets, tet_k, tbound, ranges = [], [], [], [], [], [], [], [], []
+    n_vert = 0
+    static_anim = []
+
+    def place(v, mesh_id, m, tet):
+        if mesh_id in center_ids:
+            v = v - v.mean(axis=0)
+        scale = float(_get(m, "t.scale", 1.0))
+        R = euler_xyz_degrees(*[float(a) for a in _get(m, "t.rotate", [0, 0, 0])])
+        # tri/static: scale then rotate; tet: rotate then scale (same for a uniform scale)
+        v = (v @ R.T) * scale if tet else (v * scale) @ R.T
+        return v + np.asarray(_get(m, "t.translate", [0, 0, 0]), np.float64)
+
+    order = [i for i, m in enumerate(meshes) if _get(m, "f.type") in ("tri", "tet")]
+    order += [i for i, m in enumerate(meshes) if _get(m, "f.type") == "static"]
+    n_dyn = None
+    for mesh_id, i in enumerate(order):
+        m = meshes[i]
+        mtype = _get(m, "f.type")
+        if mtype == "static" and n_dyn is None:
+            n_dyn = n_vert
+        path = os.path.join(resource_dir, _get(m, "f.path").replace("\\", "/"))
+        if mtype == "tet":
+            v, t = read_msh(path)
+            v = place(v, mesh_id, m, True)
+            tets.append(t + n_vert)
+            tet_k.append(np.full(len(t), float(_get(m, "k.stretch", 0.0))))
+            tbound.append(boundary_facets(t) + n_vert)
+            bend_k.append(np.zeros(len(v)))
+            dens.append(np.full(len(v), float(m.get("density", 1.0))))
+        else:
+            v, f = read_obj(path)
+            v = place(v, mesh_id, m, False)
+            tris.append((f + n_vert, mtype))
+            if mtype == "tri":
+                tri_k.append(np.full(len(f), float(_get(m, "k.stretch", 0.0))))
+                bend_k.append(np.full(len(v), float(_get(m, "k.bending", 0.0))))
+                dens.append(np.full(len(v), float(m.get("density", 1.0))))
+            else:
+                bend_k.append(np.zeros(len(v)))
+                dens.append(np.full(len(v), STATIC_DENSITY))
+                anim = m.get("animate", "")
+                if anim:
+                    static_anim.append((n_vert, len(v), os.path.join(resource_dir, anim.replace("\\", "/"))))
+        xs.append(v)
+        ranges.append((n_vert, len(v), mtype))
+        n_vert += len(v)
+    if n_dyn is None:
+        n_dyn = n_vert
+    x = np.concatenate(xs) if xs else np.zeros((0, 3))
+    # faces: dynamic cloth faces first, then static faces (order of the vertex ranges)
+    dyn_f = [f for f, t in tris if t == "tri"]
+    st_f = [f for f, t in tris if t == "static"]
+    all_f = np.concatenate(dyn_f + st_f) if (dyn_f or st_f) else np.zeros((0, 3), np.int32)
+    tk = np.concatenate(tri_k + [np.zeros(sum(len(f) for f in st_f))]) if all_f.size else np.zeros(0)
+    pins = [int(p) for p in _get(cfg, "global.pin", []) if int(p) < n_dyn]
+    scene = Scene.from_mesh(x, n_dyn, np.concatenate(dens), tris=all_f, tri_k=tk, bend_k=np.concatenate(bend_k),
+                            tets=np.concatenate(tets) if tets else None, tet_k=np.concatenate(tet_k) if tet_k else None,
+                            tet_boundary=np.concatenate(tbound) if tbound else None, pins=np.array(pins, np.int32))
+
+    sub_step = int(_get(cfg, "global.sub_step", 1))
+    c = AdmmConfig()
+    c.dt = float(_get(cfg, "global.dt", c.dt))
+    c.g = float(_get(cfg, "global.g", c.g))
+    c.admm_max_iter = int(_get(cfg, "solver.admm_max_iter", c.admm_max_iter))
+    c.dcd_interval = int(_get(cfg, "solver.DCD_interval", c.dcd_interval))
+    c.gs_max_iter = int(_get(cfg, "solver.GS_max_iter", c.gs_max_iter))
+    c.warmstart_ue = bool(_get(cfg, "solver.admm.warmstart_Ue", c.warmstart_ue))
+    c.warmstart_uc = bool(_get(cfg, "solver.admm.warmstart_Uc", c.warmstart_uc))
+    c.enable_contact = bool(_get(cfg, "solver.contact.enable", c.enable_contact))
+    c.w_scale = float(_get(cfg, "solver.contact.w_scale", c.w_scale))
+    c.kappa = float(_get(cfg, "solver.contact.kappa", c.kappa))
+    c.beta = float(_get(cfg, "solver.contact.beta", c.beta))
+    c.mu = float(_get(cfg, "solver.contact.mu", c.mu))
+    c.damp_kl = float(_get(cfg, "solver.damp.kl", c.damp_kl))
+    c.damp_km = float(_get(cfg, "solver.damp.km", c.damp_km))
+    c.thickness = float(_get(cfg, "DCD.narrow.thickness", c.thickness))
+    c.max_contacts = int(_get(cfg, "DCD.narrow.max_collision", c.max_contacts))
+
+    tracks = []
+    k = len(static_anim)
+    for j, (first, count, npz) in enumerate(static_anim):
+        if "sequence" in os.path.basename(npz):
+            raise NotImplementedError(f"{npz}: point-sequence animation is not supported")
+        with np.load(npz, allow_pickle=False) as d:
+            pos = np.asarray(d["pos"], np.float64).reshape(-1, 3)
+            quat = np.asarray(d["quat"], np.float64).reshape(-1, 4)
+        tracks.append((first, count, pos[j:], quat[j:], k + 1))
+    return SceneSpec(scene=scene, config=c, tracks=tracks, scene_name=str(cfg.get("scene_name", "scene")),
+                     end_frame=int(cfg.get("end_frame", 2000)), sub_step=sub_step, mesh_ranges=ranges)
+
+
+# ---------------------------------------------------------------------------
+# Frame export and metrics
+# ---------------------------------------------------------------------------
+def frame_name(scene_name: str, step_idx: int) -> str:
+    """<scene>_<NNNNN> (export_frame.h:45 writes "%s\\_%05zu", a path
+    separator on the reference's Windows build; compare_solvers.py matches
+    r"_(\\d{5})\\.obj$")."""
+    return f"{scene_name}_{step_idx:05d}"
+
+
+def export_frame(out_dir: str, scene_name: str, step_idx: int, x: np.ndarray, tris: np.ndarray, binary: bool = False) -> str:
+    """Writes all vertices and the surface triangles as OBJ, or as binary
+    little-endian PLY (float32 vertices, int32 faces) when `binary`."""
+    os.makedirs(out_dir, exist_ok=True)
+    x = np.asarray(x, np.float64).reshape(-1, 3)
+    tris = np.asarray(tris, np.int64).reshape(-1, 3)
+    base = os.path.join(out_dir, frame_name(scene_name, step_idx))
+    if not binary:
+        path = base + ".obj"
+        with open(path, "w") as f:
+            f.writelines(f"v {a:.9g} {b:.9g} {c:.9g}\n" for a, b, c in x)
+            f.writelines(f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in tris)
+        return path
+    path = base + ".ply"
+    with open(path, "wb") as f:
+        f.write((f"ply\nformat binary_little_endian 1.0\nelement vertex {len(x)}\nproperty float x\n"
+                 f"property float y\nproperty float z\nelement face {len(tris)}\n"
+                 f"property list uchar int vertex_indices\nend_header\n").encode("ascii"))
+        f.write(x.astype("<f4").tobytes())
+        rec = np.zeros(len(tris), dtype=[("n", "u1"), ("i", "<i4", (3,))])
+        rec["n"] = 3
+        rec["i"] = tris
+        f.write(rec.tobytes())
+    return path
+
+
+def read_frame(path: str) -> np.ndarray:
+    """Vertex positions of a frame written by export_frame (OBJ or binary PLY)."""
+    if path.endswith(".obj"):
+        return read_obj(path)[0]
+    with open(path, "rb") as f:
+        data = f.read()
+    head, body = data.split(b"end_header\n", 1)
+    nv = int(re.search(rb"element vertex (\d+)", head).group(1))
+    return np.frombuffer(body[: 12 * nv], "<f4").reshape(nv, 3).astype(np.float64)
+
+
+def write_step_metrics_csv(path: str, metrics) -> None:
+    """step_metrics.csv: header frame,step_ms,n_contacts (solver.h:38-45)."""
+    with open(path, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["frame", "step_ms", "n_contacts"])
+        for frame, ms, nc in metrics:
+            w.writerow([int(frame), float(ms), int(nc)])
+
+
+def _load_frames(run_dir: str) -> dict:
+    frames = {}
+    for path in sorted(glob.glob(os.path.join(run_dir, "**", "*"), recursive=True)):
+        m = re.search(r"_(\d{5})\.(obj|ply)$", path)
+        if m:
+            frames[int(m.group(1))] = read_frame(path)
+    return frames
+
+
+def _load_metrics(run_dir: str) -> dict:
+    path = os.path.join(run_dir, "step_metrics.csv")
+    if not os.path.exists(path):
+        return {}
+    with open(path, newline="") as f:
+        return {int(r["frame"]): (float(r["step_ms"]), int(r["n_contacts"])) for r in csv.DictReader(f)}
+
+
+def compare_runs(run_a: str, run_b: str, out_csv: str, names=("a", "b")) -> list:
+    """Per common frame: mean and max vertex distance, contacts and step_ms of
+    each run (the frames.csv of compare_solvers.py:143-198), plus
+    rel_err = ||x_a - x_b|| / ||x_b - x_b(first common frame)|| (0 when the
+    denominator is 0).  Returns the rows."""
+    fa, fb = _load_frames(run_a), _load_frames(run_b)
+    ma, mb = _load_metrics(run_a), _load_metrics(run_b)
+    common = sorted(set(fa) & set(fb))
+    if not common:
+        raise ValueError("no common frames to compare")
+    x0 = fb[common[0]]
+    rows = []
+    for fr in common:
+        a, b = fa[fr], fb[fr]
+        n = min(len(a), len(b))
+        d = np.linalg.norm(a[:n] - b[:n], axis=1)
+        den = np.linalg.norm(b[:n] - x0[:n])
+        rel = float(np.linalg.norm(a[:n] - b[:n]) / den) if den > 0 else 0.0
+        qa, qb = ma.get(fr, ("", "")), mb.get(fr, ("", ""))
+        rows.append((fr, float(d.mean()) if n else 0.0, float(d.max()) if n else 0.0, qa[1], qb[1], qa[0], qb[0], rel))
+    with open(out_csv, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(["frame", "mean_vertex_diff", "max_vertex_diff", f"contacts_{names[0]}", f"contacts_{names[1]}",
+                    f"step_ms_{names[0]}", f"step_ms_{names[1]}", "rel_err"])
+        w.writerows(rows)
+    return rows
+
+
+# ---------------------------------------------------------------------------
+# Headless frame loop
+# ---------------------------------------------------------------------------
+def run_scene(spec: SceneSpec, out_dir: str, frames: int | None = None, export: bool = True, binary: bool = False,
+              device: int = 0) -> list:
+    """The reference's frame loop without the GUI (main.cpp:29-94): per frame
+    `sub_step` solver steps, then the frame is exported as
+    <scene>_<frame>.obj|.ply; step_metrics.csv gets one row per solver step.
+    Returns the metrics rows (frame, step_ms, n_contacts)."""
+    from . import ADMMBackendHIP
+
+    b = ADMMBackendHIP.from_scene(spec.scene, spec.config, device=device)
+    for first, count, pos, quat, fps in spec.tracks:
+        b.add_rigid_track(first, count, pos, quat, frames_per_step=fps)
+    b.set_sync_to_host(export)
+    tris = spec.scene.topology()["surf_tris"]
+    n_frames = spec.end_frame + 1 if frames is None else frames
+    metrics, step = [], 0
+    os.makedirs(out_dir, exist_ok=True)
+    for fr in range(n_frames):
+        for _ in range(spec.sub_step):
+            st = b.step()
+            metrics.append((step, st["step_ms"], st["n_contacts"]))
+            step += 1
+        if export:
+            export_frame(out_dir, spec.scene_name, fr, b.get_state()[0], tris, binary)
+    write_step_metrics_csv(os.path.join(out_dir, "step_metrics.csv"), metrics)
+    return metrics
+
+
+def main(argv=None) -> int:
+    import argparse
+
+    ap = argparse.ArgumentParser(description="run a scene TOML headless on the GPU and export frames")
+    ap.add_argument("scene")
+    ap.add_argument("resource")
+    ap.add_argument("--out", default="out")
+    ap.add_argument("--frames", type=int, default=None)
+    ap.add_argument("--ply", action="store_true")
+    a = ap.parse_args(argv)
+    spec = load_scene(a.scene, a.resource)
+    m = run_scene(spec, a.out, a.frames, True, a.ply)
+    print(f"{spec.scene_name}: {len(m)} steps, mean step {np.mean([r[1] for r in m]):.2f} ms")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

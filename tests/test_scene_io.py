@@ -1,0 +1,288 @@
+"""Scene files, frame export and run comparison (parallel_admm_amd/scene_io.py).
+CPU only: readers on files written here, the TOML loader on a scene in the
+reference's schema, export round trips, compare_runs on runs with known
+differences, and (when the reference checkout is present) its own scene
+files."""
+import os
+
+import numpy as np
+import pytest
+
+from parallel_admm_amd import scene_io as sio
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+REFERENCE = "/root/reference"
+
+
+def _write_obj(path, v, f):
+    with open(path, "w") as fh:
+        fh.writelines(f"v {a} {b} {c}\n" for a, b, c in v)
+        fh.writelines(f"f {a + 1} {b + 1} {c + 1}\n" for a, b, c in f)
+
+
+def _cube_tets():
+    """Unit cube split into 5 tets (positively oriented)."""
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0], [0, 0, 1], [1, 0, 1], [0, 1, 1], [1, 1, 1]], float)
+    t = np.array([[0, 1, 2, 4], [1, 3, 2, 7], [1, 5, 4, 7], [2, 4, 7, 6], [1, 2, 4, 7]], np.int32)
+    for k, (a, b, c, d) in enumerate(t):
+        if np.linalg.det(np.array([v[b] - v[a], v[c] - v[a], v[d] - v[a]])) < 0:
+            t[k, [2, 3]] = t[k, [3, 2]]
+    return v, t
+
+
+def _write_msh41(path, v, t):
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n4.1 0 8\n$EndMeshFormat\n$Nodes\n")
+        f.write(f"1 {len(v)} 1 {len(v)}\n3 1 0 {len(v)}\n")
+        f.writelines(f"{i + 1}\n" for i in range(len(v)))
+        f.writelines(f"{a} {b} {c}\n" for a, b, c in v)
+        f.write("$EndNodes\n$Elements\n")
+        f.write(f"2 {len(t) + 1} 1 {len(t) + 1}\n2 1 2 1\n1 1 2 3\n3 1 4 {len(t)}\n")
+        f.writelines(f"{k + 2} {a + 1} {b + 1} {c + 1} {d + 1}\n" for k, (a, b, c, d) in enumerate(t))
+        f.write("$EndElements\n")
+
+
+def _write_msh22(path, v, t):
+    with open(path, "w") as f:
+        f.write("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n")
+        f.write(f"{len(v)}\n")
+        f.writelines(f"{i + 1} {a} {b} {c}\n" for i, (a, b, c) in enumerate(v))
+        f.write(f"$EndNodes\n$Elements\n{len(t) + 1}\n1 2 2 0 1 1 2 3\n")
+        f.writelines(f"{k + 2} 4 2 0 1 {a + 1} {b + 1} {c + 1} {d + 1}\n" for k, (a, b, c, d) in enumerate(t))
+        f.write("$EndElements\n")
+
+
+def test_read_obj_matches_reference_grid(tmp_path):
+    g = np.load(os.path.join(GOLDEN, "reference_grids.npz"))
+    p = tmp_path / "g.obj"
+    _write_obj(p, g["grid20_v"], g["grid20_f"])
+    v, f = sio.read_obj(str(p))
+    np.testing.assert_array_equal(v, g["grid20_v"])
+    np.testing.assert_array_equal(f, g["grid20_f"])
+    (tmp_path / "quad.obj").write_text("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nf 1/1 2/2 3/3 4/4\n")
+    np.testing.assert_array_equal(sio.read_obj(str(tmp_path / "quad.obj"))[1], [[0, 1, 2], [0, 2, 3]])
+
+
+def _write_msh22_binary(path, v, t):
+    import struct
+    with open(path, "wb") as f:
+        f.write(b"$MeshFormat\n2.2 1 8\n" + struct.pack("<i", 1) + b"\n$EndMeshFormat\n$Nodes\n")
+        f.write(f"{len(v)}\n".encode())
+        for i, p in enumerate(v):
+            f.write(struct.pack("<i3d", i + 1, *p))
+        f.write(f"\n$EndNodes\n$Elements\n{len(t) + 1}\n".encode())
+        f.write(struct.pack("<3i", 2, 1, 2) + struct.pack("<6i", 1, 0, 1, 1, 2, 3))  # one triangle, 2 tags
+        f.write(struct.pack("<3i", 4, len(t), 2))
+        for k, tt in enumerate(t):
+            f.write(struct.pack("<7i", k + 2, 0, 1, *(int(a) + 1 for a in tt)))
+        f.write(b"\n$EndElements\n")
+
+
+@pytest.mark.parametrize("writer", [_write_msh41, _write_msh22, _write_msh22_binary], ids=["msh4.1", "msh2.2", "msh2.2-binary"])
+def test_read_msh_and_boundary(tmp_path, writer):
+    v, t = _cube_tets()
+    p = tmp_path / "cube.msh"
+    writer(p, v, t)
+    vr, tr = sio.read_msh(str(p))
+    np.testing.assert_array_equal(vr, v)
+    np.testing.assert_array_equal(tr, t)
+    b = sio.boundary_facets(tr)
+    assert len(b) == 12  # two triangles per cube face
+    # closed surface: every edge is shared by exactly two boundary triangles, and outward normals
+    edges = {}
+    for f in b:
+        for a, c in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
+            edges[(min(a, c), max(a, c))] = edges.get((min(a, c), max(a, c)), 0) + 1
+    assert set(edges.values()) == {2}
+    ctr = v.mean(0)
+    for f in b:
+        n = np.cross(v[f[1]] - v[f[0]], v[f[2]] - v[f[0]])
+        assert n @ (v[f].mean(0) - ctr) > 0
+
+
+def test_euler_rotation_order():
+    # Rz Ry Rx: a 90 degree x rotation maps y to z (up to the reference's pi)
+    R = sio.euler_xyz_degrees(90, 0, 0)
+    np.testing.assert_allclose(R @ [0, 1, 0], [0, 0, 1], atol=1e-7)
+    R = sio.euler_xyz_degrees(90, 90, 0)  # x first, then y
+    np.testing.assert_allclose(R @ [0, 1, 0], [1, 0, 0], atol=1e-7)
+    np.testing.assert_allclose(R.T @ R, np.eye(3), atol=1e-14)
+
+
+SCENE = """scene_name = "unit"
+end_frame = 3
+[global]
+dt = 0.01
+pin = [0, 4, 10000]
+g = 9.8
+sub_step = 2
+[solver]
+admm_max_iter = 17
+DCD_interval = 4
+GS_max_iter = 3
+admm.warmstart_Ue = true
+admm.warmstart_Uc = false
+contact.enable = true
+contact.w_scale = 20
+contact.kappa = 2e6
+contact.beta = 5e5
+contact.mu = 0.25
+damp.kl = 1e-5
+damp.km = 0
+[DCD]
+narrow.max_collision = 4096
+narrow.thickness = 2e-3
+[[mesh]]
+f.path = 'floor.obj'
+f.type = 'static'
+t.scale = 2.0
+t.translate = [0, -0.5, 0]
+t.rotate = [0, 0, 0]
+animate = 'spin.npz'
+[[mesh]]
+f.path = 'cloth.obj'
+f.type = 'tri'
+t.scale = 0.5
+t.translate = [0, 1, 0]
+t.rotate = [90, 0, 0]
+k.stretch = 800.0
+k.bending = 0.5
+density = 3.0
+[[mesh]]
+f.path = 'cube.msh'
+f.type = 'tet'
+t.scale = 0.1
+t.translate = [1, 1, 1]
+t.rotate = [0, 0, 0]
+k.stretch = 5e4
+density = 1000.0
+[mesh_post]
+center_ids = [0]
+"""
+
+
+def test_load_scene_toml(pa, tmp_path):
+    g = np.load(os.path.join(GOLDEN, "reference_grids.npz"))
+    _write_obj(tmp_path / "cloth.obj", g["grid5_v"], g["grid5_f"])
+    _write_obj(tmp_path / "floor.obj", np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1.0]]),
+               np.array([[0, 2, 1], [0, 3, 2]]))
+    v, t = _cube_tets()
+    _write_msh41(tmp_path / "cube.msh", v, t)
+    np.savez(tmp_path / "spin.npz", pos=np.zeros((5, 3)), quat=np.tile([0, 0, 0, 1.0], (5, 1)))
+    (tmp_path / "s.toml").write_text(SCENE)
+    spec = sio.load_scene(str(tmp_path / "s.toml"), str(tmp_path))
+    info = spec.scene.info
+    # order: dynamic tri, dynamic tet, then the static floor
+    assert [r[2] for r in spec.mesh_ranges] == ["tri", "tet", "static"]
+    assert (info["n_dyn"], info["n_all"]) == (25 + 8, 25 + 8 + 4)
+    m = spec.scene.mesh()
+    cloth = m["x"][:25]
+    # cloth: mesh id 0 is centred (mesh_post), scaled 0.5, rotated 90 deg about x, lifted by 1
+    c = g["grid5_v"] - g["grid5_v"].mean(0)
+    np.testing.assert_allclose(cloth, 0.5 * c @ sio.euler_xyz_degrees(90, 0, 0).T + [0, 1, 0], atol=1e-12)
+    np.testing.assert_allclose(m["x"][25:33], 0.1 * v + 1.0, atol=1e-12)
+    np.testing.assert_allclose(m["x"][33:], 2.0 * np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1.0]]) + [0, -0.5, 0])
+    # densities and stiffness
+    np.testing.assert_array_equal(m["density"][:25], 3.0)
+    np.testing.assert_array_equal(m["density"][25:33], 1000.0)
+    np.testing.assert_array_equal(m["density"][33:], 1e12)
+    assert info["n_tri_c"] == 32 and info["n_tet_c"] == 5 and info["n_tet_boundary"] == 12
+    np.testing.assert_array_equal(m["tri_k"][:32], 800.0)
+    np.testing.assert_array_equal(m["tri_k"][32:], 0.0)
+    np.testing.assert_array_equal(m["pins"], [0, 4])  # vertex 10000 does not exist
+    # solver keys
+    c = spec.config
+    assert (c.dt, c.g, c.admm_max_iter, c.dcd_interval, c.gs_max_iter) == (0.01, 9.8, 17, 4, 3)
+    assert (c.warmstart_ue, c.warmstart_uc, c.kappa, c.beta, c.mu, c.w_scale) == (True, False, 2e6, 5e5, 0.25, 20.0)
+    assert (c.thickness, c.max_contacts, c.damp_kl) == (2e-3, 4096, 1e-5)
+    assert (spec.sub_step, spec.end_frame, spec.scene_name) == (2, 3, "unit")
+    # one rigid track: starts at frame 0, 2 frames per step (one rigid mesh)
+    assert len(spec.tracks) == 1
+    first, count, pos, quat, fps = spec.tracks[0]
+    assert (first, count, len(pos), fps) == (33, 4, 5, 2)
+
+
+def test_export_roundtrip_and_metrics(tmp_path):
+    x = np.random.default_rng(1).standard_normal((7, 3))
+    f = np.array([[0, 1, 2], [2, 3, 4], [4, 5, 6]])
+    p1 = sio.export_frame(str(tmp_path), "sc", 12, x, f)
+    p2 = sio.export_frame(str(tmp_path), "sc", 13, x, f, binary=True)
+    assert os.path.basename(p1) == "sc_00012.obj" and os.path.basename(p2) == "sc_00013.ply"
+    np.testing.assert_allclose(sio.read_frame(p1), x, rtol=1e-8)
+    np.testing.assert_allclose(sio.read_frame(p2), x.astype(np.float32))
+    np.testing.assert_array_equal(sio.read_obj(p1)[1], f)
+    sio.write_step_metrics_csv(str(tmp_path / "step_metrics.csv"), [(0, 1.5, 3), (1, 2.5, 4)])
+    assert (tmp_path / "step_metrics.csv").read_text().splitlines() == ["frame,step_ms,n_contacts", "0,1.5,3", "1,2.5,4"]
+
+
+def test_compare_runs_known_differences(tmp_path):
+    rng = np.random.default_rng(2)
+    x0 = rng.standard_normal((10, 3))
+    f = np.array([[0, 1, 2]])
+    a, b = tmp_path / "a", tmp_path / "b"
+    for fr in range(4):
+        xb = x0 + 0.1 * fr
+        sio.export_frame(str(b), "s", fr, xb, f)
+        xa = xb.copy()
+        xa[3] += [0.0, 0.003 * fr, 0.004 * fr]  # one vertex off by 0.005 fr
+        sio.export_frame(str(a), "s", fr, xa, f)
+    sio.export_frame(str(a), "s", 9, x0, f)  # not in b: ignored
+    sio.write_step_metrics_csv(str(a / "step_metrics.csv"), [(fr, 1.0 + fr, fr) for fr in range(4)])
+    rows = sio.compare_runs(str(a), str(b), str(tmp_path / "frames.csv"), names=("hip", "cpu"))
+    assert [r[0] for r in rows] == [0, 1, 2, 3]
+    for fr, mean, mx, ca, cb, ma, mb, rel in rows:
+        assert mx == pytest.approx(0.005 * fr, abs=1e-7)
+        assert mean == pytest.approx(0.005 * fr / 10, abs=1e-8)
+        assert ca == fr and ma == 1.0 + fr and cb == ""
+        if fr:
+            assert rel == pytest.approx(0.005 * fr / np.sqrt(30 * (0.1 * fr) ** 2), rel=1e-6)
+    header = (tmp_path / "frames.csv").read_text().splitlines()[0]
+    assert header == "frame,mean_vertex_diff,max_vertex_diff,contacts_hip,contacts_cpu,step_ms_hip,step_ms_cpu,rel_err"
+
+
+@pytest.mark.skipif(not os.path.isdir(os.path.join(REFERENCE, "resource", "scene")), reason="reference checkout absent")
+@pytest.mark.parametrize("name", ["grid.toml", "card.toml", "cylinder.toml"])
+def test_reference_scene_files_load(pa, name):
+    """The reference's own scene files build (topology, constraints, mass) unchanged."""
+    path = os.path.join(REFERENCE, "resource", "scene", name)
+    if not os.path.exists(path):
+        pytest.skip(f"{name} not in this checkout")
+    try:
+        spec = sio.load_scene(path, os.path.join(REFERENCE, "resource"))
+    except NotImplementedError as e:
+        pytest.skip(str(e))
+    i = spec.scene.info
+    assert i["n_dyn"] > 0 and i["m_rows"] > 0
+    assert spec.config.dt > 0 and spec.config.max_contacts > 0
+    assert np.all(np.isfinite(spec.scene.mass()))
+
+
+@pytest.mark.gpu
+def test_run_scene_exports_frames_and_metrics(pa, tmp_path):
+    """The headless loop on the GPU: a cloth (the reference's 5x5 grid, scaled)
+    dropped on a static floor from a scene TOML; frames and step_metrics.csv
+    are written, two runs give identical frames (compare_runs reports zero
+    difference) and the cloth lands on the floor."""
+    g = np.load(os.path.join(GOLDEN, "reference_grids.npz"))
+    _write_obj(tmp_path / "cloth.obj", g["grid5_v"], g["grid5_f"])
+    _write_obj(tmp_path / "floor.obj", np.array([[-1, 0, -1], [1, 0, -1], [1, 0, 1], [-1, 0, 1.0]]),
+               np.array([[0, 2, 1], [0, 3, 2]]))
+    v, t = _cube_tets()
+    _write_msh41(tmp_path / "cube.msh", v, t)
+    np.savez(tmp_path / "spin.npz", pos=np.zeros((8, 3)), quat=np.tile([0, 0, 0, 1.0], (8, 1)))
+    scene = SCENE.replace("t.translate = [0, -0.5, 0]", "t.translate = [0, 0, 0]").replace(
+        "t.translate = [0, 1, 0]", "t.translate = [0, 0.003, 0]").replace(
+        "t.rotate = [90, 0, 0]", "t.rotate = [0, 0, 0]").replace("t.scale = 0.5", "t.scale = 0.1")
+    (tmp_path / "s.toml").write_text(scene)
+    spec = sio.load_scene(str(tmp_path / "s.toml"), str(tmp_path))
+    runs = []
+    for name in ("r1", "r2"):
+        out = tmp_path / name
+        m = sio.run_scene(spec, str(out), frames=4)
+        assert len(m) == 4 * spec.sub_step
+        assert sorted(p.name for p in out.glob("*.obj")) == [f"unit_{k:05d}.obj" for k in range(4)]
+        assert (out / "step_metrics.csv").read_text().startswith("frame,step_ms,n_contacts\n")
+        runs.append(m)
+    rows = sio.compare_runs(str(tmp_path / "r1"), str(tmp_path / "r2"), str(tmp_path / "frames.csv"))
+    assert all(r[2] == 0.0 for r in rows)
+    assert max(r[2] for r in runs[0]) > 0  # contacts with the floor
