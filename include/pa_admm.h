@@ -71,7 +71,7 @@ typedef struct pa_admm_config {
     int32_t max_contacts;      /* narrow.max_collision: contact capacity */
     int32_t global_refine;     /* refinement passes per global solve (FP64 residual, FP32 PCG correction) */
     int32_t precond;           /* PCG preconditioner: 0 Jacobi, 1 smoothed-aggregation AMG V-cycle */
-    int32_t reserved;
+    int32_t graphs;            /* 1: replay PCG iteration pairs as HIP graphs; 0: launch every kernel (default) */
 } pa_admm_config;
 
 /* Fills the reference's ADMMSolverConfig defaults (admm_config.h:20-53). */

@@ -55,7 +55,7 @@ class AdmmConfigC(C.Structure):
         ("mu", C.c_double), ("kappa", C.c_double), ("beta", C.c_double), ("thickness", C.c_double),
         ("damp_kl", C.c_double), ("damp_km", C.c_double), ("w_scale", C.c_double), ("residual_tol", C.c_double),
         ("residual_interval", C.c_int32), ("global_max_iter", C.c_int32), ("global_tol", C.c_double),
-        ("max_contacts", C.c_int32), ("global_refine", C.c_int32), ("precond", C.c_int32), ("reserved", C.c_int32),
+        ("max_contacts", C.c_int32), ("global_refine", C.c_int32), ("precond", C.c_int32), ("graphs", C.c_int32),
     ]
 
 
@@ -151,7 +151,7 @@ class AdmmConfig:
     max_contacts: int = 1 << 20
     global_refine: int = 1
     precond: int = 1
-    reserved: int = 0
+    graphs: int = 0
 
     def to_c(self) -> AdmmConfigC:
         c = AdmmConfigC()

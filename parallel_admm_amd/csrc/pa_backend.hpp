@@ -146,6 +146,11 @@ struct Backend {
     void add_rigid_track(int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps);
     void animate_tracks(bool place_first_frame);
     int global_solve(int max_iter, double tol);
+    void pcg_iteration(int it, bool fused, int& pc, int& rc);
+    hipGraphExec_t pcg_pair_graph(bool first_pair);
+    hipGraphExec_t pcg_graph[2] = {nullptr, nullptr}; // [0] later pairs, [1] the first pair of a solve
+    bool capturing = false;                           // inside a stream capture: no event records
+    bool use_graphs = true;
     // cg_z = M^-1 r, partial r.z into cg_part[*, 6..8]; with skip_pre0 the
     // level-0 pre-smoothing was already done by k_cg_update_pre.
     void precondition(const float* r, bool skip_pre0 = false);

@@ -81,6 +81,7 @@ void pa_admm_config_default(pa_admm_config* c) {
     c->max_contacts = 1 << 20;
     c->global_refine = 1;
     c->precond = 1;
+    c->graphs = 0;
 }
 
 // ---------------------------------------------------------------- scene
@@ -251,6 +252,8 @@ pa_status pa_detector_detect(pa_detector* d, const double* xh, int32_t* n_contac
         D.x_own.reserve(xf.size());
         D.x_own.upload(xf.data(), xf.size(), D.stream);
         int nc = D.detect(D.x_own.p);
+        // ICollisionDetector::detect is synchronous: the sorted list is complete on return
+        PA_HIP(hipStreamSynchronize(D.stream));
         if (n_contacts) *n_contacts = nc;
     });
 }

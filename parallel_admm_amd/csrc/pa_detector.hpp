@@ -41,10 +41,12 @@ struct Detector {
     DBuf<int32_t> surf_verts, tris, edges;
     DBuf<float> x_own; // positions uploaded through detect(host x)
     Tree ft, et;
-    ContactBuffers c;
+    ContactBuffers c;   // canonical list
+    ContactBuffers stg; // staging list in discovery order (single traversal)
+    DBuf<int32_t> st_seg; // segment of each staged contact
     DBuf<uint32_t> qcount, qoffset; // nv + ne segment counts / offsets (PT per query vertex, EE per smaller edge id)
-    DBuf<uint32_t> qfill;           // ne: EE slots taken per segment during the write pass
-    DBuf<uint32_t> total;           // [0] = contacts found (may exceed cap), [1] = stack overflow flag
+    DBuf<uint32_t> qfill;           // nv + ne: slots taken per segment by the scatter
+    DBuf<uint32_t> total;           // [0] contacts found (segment counts), [1] stack overflow flag, [2] staged contacts
     PrimScratch scratch;
     int n_contacts = 0, peak = 0;
     int64_t found_total = 0; // contacts found by the last detection (may exceed cap)
@@ -54,8 +56,10 @@ struct Detector {
              double dhat, hipStream_t s);
     ~Detector();
     // Runs detection on device positions x (3 n_all floats, on this->stream).
-    // Returns the number of contacts stored; sets found_total and throws
-    // PA_ERR_OVERFLOW after storing `cap` contacts if more were found.
+    // Returns the number of contacts; the sorted list is complete once the
+    // work queued on this->stream has run (the last kernels are queued after
+    // the host reads the count).  Sets found_total and throws PA_ERR_OVERFLOW
+    // (list unspecified) if more than `cap` contacts were found.
     int detect(const float* x);
     void set_stream(hipStream_t s) { stream = s; }
 

@@ -290,18 +290,32 @@ __device__ inline void sort_segment(const QueryOut& o, int begin, int end) {
     }
 }
 
-// Point-triangle queries: one lane per surface vertex, LDS stack per lane.
-// write == 0: count accepted pairs into qcount; write == 1: store them at
-// qoffset and sort the segment by face id.
-__global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __restrict__ x, const int32_t* __restrict__ surf_verts,
-                                                   int nv, const int32_t* __restrict__ tris, int n_dyn, int nf,
-                                                   const float4* __restrict__ rec, const uint32_t* __restrict__ ids, double dhat,
-                                                   float gap, uint32_t* __restrict__ qcount, const uint32_t* __restrict__ qoffset,
-                                                   QueryOut o, uint32_t* __restrict__ flags) {
+// Slot in the staging list for every lane of the wave that reaches this call
+// together (one atomic per wave and call site; lanes that diverged away are
+// not counted).
+__device__ inline int stage_slot(uint32_t* __restrict__ counter) {
+    const unsigned long long m = __ballot(1);
+    const int lane = __lane_id();
+    const int leader = __ffsll(m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return (int)(base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)));
+}
+
+// Point-triangle queries: one lane per surface vertex q, LDS stack per lane.
+// Accepted pairs go to the staging list (any order) with segment q; the count
+// per vertex goes to qcount[q].
+__global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, const int32_t* __restrict__ surf_verts, int nv,
+                                                   const int32_t* __restrict__ tris, int n_dyn, int nf, const float4* __restrict__ rec,
+                                                   const uint32_t* __restrict__ ids, double dhat, float gap, uint32_t* __restrict__ qcount,
+                                                   QueryOut st, int32_t* __restrict__ st_seg, uint32_t* __restrict__ counter,
+                                                   uint32_t* __restrict__ flags) {
     __shared__ int32_t stack[STACK][QBLK];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nv || nf == 0) {
-        if (q < nv && !write) qcount[q] = 0;
+    if (q >= nv) return;
+    if (nf == 0) {
+        qcount[q] = 0;
         return;
     }
     const int v = surf_verts[q];
@@ -310,7 +324,6 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __res
     const bool vstatic = v >= n_dyn;
     const int leaf0 = nf - 1;
     int count = 0;
-    const int base = write ? (int)qoffset[q] : 0;
     auto leaf = [&](int node) {
         const int f = (int)ids[node - leaf0];
         const int a = tris[3 * f], b = tris[3 * f + 1], c = tris[3 * f + 2];
@@ -318,9 +331,10 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __res
         const dv3 t0 = ldd(x, a), t1 = ldd(x, b), t2 = ldd(x, c);
         const PTOut r = point_triangle(p, t0, t1, t2);
         if (!pt_accept(r, dhat)) return;
-        if (write) {
-            const int idx = base + count;
-            if (idx < o.cap) write_contact(o, idx, 0, v, f, make_int4(v, a, b, c), pt_fields(p, t0, t1, t2, r, dhat), r.dist);
+        const int idx = stage_slot(counter);
+        if (idx < st.cap) {
+            write_contact(st, idx, 0, v, f, make_int4(v, a, b, c), pt_fields(p, t0, t1, t2, r, dhat), r.dist);
+            st_seg[idx] = q;
         }
         count++;
     };
@@ -352,24 +366,18 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(int write, const float* __res
             }
         }
     }
-    if (!write) {
-        qcount[q] = (uint32_t)count;
-    } else {
-        const int end = min(base + count, o.cap);
-        sort_segment(o, base, end);
-    }
+    qcount[q] = (uint32_t)count;
 }
 
 // Edge-edge queries: lane k takes the edge at sorted leaf k and visits only
-// leaves after k.  A pair (i, j), i < j, belongs to segment i and is evaluated
-// with edge i first, as the canonical definition does.  write == 0: count
-// into qcount[i] (zeroed beforehand); write == 1: store at
-// qoffset[i] + qfill[i]++ (qfill zeroed beforehand; k_sort_segments orders
-// each segment afterwards).
-__global__ void __launch_bounds__(QBLK) k_query_ee(int write, const float* __restrict__ x, const int32_t* __restrict__ edges, int ne,
-                                                   int n_dyn, const float4* __restrict__ rec, const uint32_t* __restrict__ ids,
-                                                   double dhat, float gap, uint32_t* __restrict__ qcount,
-                                                   const uint32_t* __restrict__ qoffset, uint32_t* __restrict__ qfill, QueryOut o,
+// leaves after k, so each unordered pair is tested once.  A pair (i, j),
+// i < j, is evaluated with edge i first, as the canonical definition does, and
+// belongs to segment i: it goes to the staging list with segment seg0 + i and
+// qcount[i] counts it (zeroed beforehand).
+__global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, const int32_t* __restrict__ edges, int ne, int n_dyn,
+                                                   const float4* __restrict__ rec, const uint32_t* __restrict__ ids, double dhat,
+                                                   float gap, uint32_t* __restrict__ qcount, int seg0, QueryOut st,
+                                                   int32_t* __restrict__ st_seg, uint32_t* __restrict__ counter,
                                                    uint32_t* __restrict__ flags) {
     __shared__ int32_t stack[STACK][QBLK];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,11 +399,11 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(int write, const float* __res
         const dv3 P0 = ldd(x, i0), P1 = ldd(x, i1), Q0 = ldd(x, j0), Q1 = ldd(x, j1);
         const EEOut r = segment_segment(P0, P1, Q0, Q1);
         if (!ee_accept(r, dhat)) return;
-        if (!write) {
-            atomicAdd(&qcount[i], 1u);
-        } else {
-            const int idx = (int)(qoffset[i] + atomicAdd(&qfill[i], 1u));
-            if (idx < o.cap) write_contact(o, idx, 1, i, j, make_int4(i0, i1, j0, j1), ee_fields(P0, P1, Q0, Q1, r, dhat), r.dist);
+        atomicAdd(&qcount[i], 1u);
+        const int idx = stage_slot(counter);
+        if (idx < st.cap) {
+            write_contact(st, idx, 1, i, j, make_int4(i0, i1, j0, j1), ee_fields(P0, P1, Q0, Q1, r, dhat), r.dist);
+            st_seg[idx] = seg0 + i;
         }
     };
     int sp = 0, node = 0;
@@ -422,6 +430,25 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(int write, const float* __res
             node = stack[--sp][threadIdx.x];
         }
     }
+}
+
+// Staged contact c goes to its segment: position qoffset[seg] + qfill[seg]++
+// (qfill zeroed beforehand); k_sort_segments then fixes the order inside each
+// segment.
+__global__ void k_scatter_contacts(int n, const int32_t* __restrict__ st_seg, QueryOut st, const uint32_t* __restrict__ qoffset,
+                                   uint32_t* __restrict__ qfill, QueryOut o) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int sg = st_seg[c];
+    const int idx = (int)(qoffset[sg] + atomicAdd(&qfill[sg], 1u));
+    if (idx >= o.cap) return;
+    o.type[idx] = st.type[c];
+    o.prim0[idx] = st.prim0[c];
+    o.prim1[idx] = st.prim1[c];
+    o.v[idx] = st.v[c];
+    o.bary[idx] = st.bary[c];
+    o.nrm[idx] = st.nrm[c];
+    o.dist[idx] = st.dist[c];
 }
 
 // Sorts each of `n` segments [qoffset[s], qoffset[s] + qcount[s]) (clamped to
@@ -481,8 +508,17 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
     c.dist.alloc(cap);
     qcount.alloc((size_t)nv + ne + 1);
     qoffset.alloc((size_t)nv + ne + 1);
-    qfill.alloc((size_t)ne + 1);
-    total.alloc(2);
+    qfill.alloc((size_t)nv + ne + 1);
+    stg.cap = cap;
+    stg.type.alloc(cap);
+    stg.prim0.alloc(cap);
+    stg.prim1.alloc(cap);
+    stg.v.alloc(cap);
+    stg.bary.alloc(cap);
+    stg.nrm.alloc(cap);
+    stg.dist.alloc(cap);
+    st_seg.alloc(cap);
+    total.alloc(3);
     PA_HIP(hipStreamSynchronize(stream));
 }
 
@@ -530,38 +566,48 @@ void Detector::build_tree(Tree& T, const float* x, bool faces) {
 int Detector::detect(const float* x) {
     build_tree(ft, x, true);
     build_tree(et, x, false);
-    fill<uint32_t>(total.p, 0u, 2, stream);
+    fill<uint32_t>(total.p, 0u, 3, stream);
     const float gap = (float)(dhat * 1.001);
     QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
-    uint32_t* ee_count = qcount.p + nv;
-    const uint32_t* ee_offset = qoffset.p + nv;
-    for (int write = 0; write < 2; write++) {
-        if (nv > 0)
-            hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, write, x, surf_verts.p, nv, tris.p, n_dyn, nf,
-                               ft.rec.p, ft.ids.p, dhat, gap, qcount.p, qoffset.p, o, total.p);
+    QueryOut st{stg.type.p, stg.prim0.p, stg.prim1.p, stg.v.p, stg.bary.p, stg.nrm.p, stg.dist.p, stg.cap};
+    // one traversal per query: pairs to the staging list (total[2] counts
+    // them), per-segment counts to qcount
+    if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
+    if (nv > 0) {
+        hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, x, surf_verts.p, nv, tris.p, n_dyn, nf, ft.rec.p,
+                           ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p);
         PA_LAUNCH_CHECK();
-        if (ne > 0) {
-            fill<uint32_t>(write ? qfill.p : ee_count, 0u, ne, stream);
-            hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, write, x, edges.p, ne, n_dyn, et.rec.p,
-                               et.ids.p, dhat, gap, ee_count, ee_offset, qfill.p, o, total.p);
-            PA_LAUNCH_CHECK();
-        }
-        if (write == 0) scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);
     }
     if (ne > 0) {
-        hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(ne, BLK)), dim3(BLK), 0, stream, ne, (const uint32_t*)ee_count, ee_offset, o);
+        hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, x, edges.p, ne, n_dyn, et.rec.p, et.ids.p, dhat,
+                           gap, qcount.p + nv, nv, st, st_seg.p, total.p + 2, total.p);
         PA_LAUNCH_CHECK();
     }
-    uint32_t h[2];
+    scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);
+    uint32_t h[3];
     PA_HIP(hipMemcpyAsync(h, total.p, sizeof(h), hipMemcpyDeviceToHost, stream));
     PA_HIP(hipStreamSynchronize(stream));
     found_total = h[0];
     stack_overflow = h[1] != 0;
-    n_contacts = (int)std::min<int64_t>(found_total, c.cap);
-    peak = std::max(peak, n_contacts);
     PA_CHECK(!stack_overflow, PA_ERR_INVALID, "LBVH traversal stack overflow");
-    if (found_total > c.cap)
+    PA_CHECK(h[2] == h[0], PA_ERR_INVALID, "contact staging count differs from the segment counts");
+    if (found_total > c.cap) {
+        n_contacts = 0;
         throw Error(PA_ERR_OVERFLOW, "contact capacity exceeded: found " + std::to_string(found_total) + " > " + std::to_string(c.cap));
+    }
+    n_contacts = (int)found_total;
+    peak = std::max(peak, n_contacts);
+    if (n_contacts > 0) {
+        // canonical order: segments in query order (PT by vertex, EE by the
+        // smaller edge id), each sorted by partner id
+        fill<uint32_t>(qfill.p, 0u, (int64_t)nv + ne, stream);
+        hipLaunchKernelGGL(k_scatter_contacts, dim3(grid_for(n_contacts, BLK)), dim3(BLK), 0, stream, n_contacts, (const int32_t*)st_seg.p,
+                           st, (const uint32_t*)qoffset.p, qfill.p, o);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_sort_segments, dim3(grid_for(nv + ne, BLK)), dim3(BLK), 0, stream, nv + ne, (const uint32_t*)qcount.p,
+                           (const uint32_t*)qoffset.p, o);
+        PA_LAUNCH_CHECK();
+    }
     return n_contacts;
 }
 

@@ -613,7 +613,7 @@ template <int LPR> __device__ inline float3 group_sum(float3 s) {
 // (ptr -> col/val -> gathered vector) instead of one per entry.  Unused slots
 // pass value 0 and column 0, an index every matrix here has (the row index
 // would not do: the prolongators have more rows than columns).
-constexpr int RU = 4;
+constexpr int RU = 3;
 template <int LPR, class F>
 __device__ inline void row_foreach(int i, int rows, int sub, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                    const float* __restrict__ val, F&& f) {
@@ -855,42 +855,110 @@ __global__ void __launch_bounds__(PBLK) k_cg_state(int slot_old, int nparts, dou
     cg_advance(part, nparts, st, slot_old, be);
 }
 
+// Up to RU entries of row i that lane `sub` of its row group owns (every
+// LPR-th), loaded before any gather; e_tail..e_end is the rest of the lane's
+// entries (rows longer than RU * LPR).  Unused slots: value 0, column 0.
+template <int LPR> struct RowSlot {
+    int c[RU];
+    float v[RU];
+    int e_tail, e_end;
+    __device__ inline void load(int i, int rows, int sub, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                                const float* __restrict__ val) {
+        int e0 = 0, e1 = 0;
+        if (i < rows) {
+            e0 = ptr[i] + sub;
+            e1 = ptr[i + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            const int e = e0 + u * LPR;
+            const bool ok = e < e1;
+            c[u] = ok ? col[e] : 0;
+            v[u] = ok ? val[e] : 0.f;
+        }
+        e_tail = e0 + RU * LPR;
+        e_end = e1;
+    }
+};
+
+__device__ inline float3 ld3_or0(const float* __restrict__ a, int i, bool ok) { return ok ? ld3(a, i) : f3(0.f, 0.f, 0.f); }
+
+// The three SpMV-type kernels of the fused PCG iteration process two rows per
+// lane group (i and i + one grid's worth of rows) and issue every load that
+// does not depend on the reduced scalars (row pointers, entries, gathers, the
+// row's own values) before reducing the previous kernel's partial sums.
+
 template <int LPR>
 __global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot_old, int nparts, double* __restrict__ st,
                                                      const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                      const float* __restrict__ val, const float* __restrict__ diag,
                                                      const float* __restrict__ z, const float* __restrict__ p_old,
                                                      float* __restrict__ p_new, float* __restrict__ q, double* __restrict__ part) {
-    float be[3] = {0.f, 0.f, 0.f};
-    if (!first) cg_advance(part, nparts, st, slot_old, be);
     const int sub = threadIdx.x % LPR;
+    const int rpb = PBLK / LPR, stride = gridDim.x * rpb;
+    float be[3] = {0.f, 0.f, 0.f};
+    bool need_be = !first;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
-        const int i = r0 + threadIdx.x / LPR;
-        float3 s = f3(0.f, 0.f, 0.f);
-        if (first) {
-            row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) { s = s + v * ld3(z, c); });
-        } else {
-            row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) {
-                const float3 zc = ld3(z, c), pc = ld3(p_old, c);
-                s = s + v * f3(zc.x + be[0] * pc.x, zc.y + be[1] * pc.y, zc.z + be[2] * pc.z);
-            });
+    auto comb = [&](float3 zc, float3 pc) { return f3(zc.x + be[0] * pc.x, zc.y + be[1] * pc.y, zc.z + be[2] * pc.z); };
+    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+        const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
+        const bool ha = ia < n, hb = ib < n;
+        RowSlot<LPR> A, B;
+        A.load(ia, n, sub, ptr, col, val);
+        B.load(ib, n, sub, ptr, col, val);
+        const float3 zia = ld3_or0(z, ia, ha), zib = ld3_or0(z, ib, hb);
+        const float3 pia = first ? zia : ld3_or0(p_old, ia, ha), pib = first ? zib : ld3_or0(p_old, ib, hb);
+        const float dia = ha ? diag[ia] : 0.f, dib = hb ? diag[ib] : 0.f;
+        float3 za[RU], zb[RU], pa[RU], pb[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            za[u] = ld3(z, A.c[u]);
+            zb[u] = ld3(z, B.c[u]);
+            pa[u] = first ? za[u] : ld3(p_old, A.c[u]);
+            pb[u] = first ? zb[u] : ld3(p_old, B.c[u]);
         }
-        s = group_sum<LPR>(s);
-        if (i < n && sub == 0) {
-            const float3 zi = ld3(z, i);
-            const float3 pi = first ? zi : [&] {
-                const float3 po = ld3(p_old, i);
-                return f3(zi.x + be[0] * po.x, zi.y + be[1] * po.y, zi.z + be[2] * po.z);
-            }();
-            s = s + diag[i] * pi;
-            st3(p_new, i, pi);
-            st3(q, i, s);
-            acc[0] += (double)pi.x * s.x;
-            acc[1] += (double)pi.y * s.y;
-            acc[2] += (double)pi.z * s.z;
+        if (need_be) { // uniform per block (first row pass)
+            cg_advance(part, nparts, st, slot_old, be);
+            need_be = false;
+        }
+        float3 sa = f3(0.f, 0.f, 0.f), sb = f3(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            sa = sa + A.v[u] * (first ? za[u] : comb(za[u], pa[u]));
+            sb = sb + B.v[u] * (first ? zb[u] : comb(zb[u], pb[u]));
+        }
+        for (int e = A.e_tail; e < A.e_end; e += LPR) {
+            const int c = col[e];
+            sa = sa + val[e] * (first ? ld3(z, c) : comb(ld3(z, c), ld3(p_old, c)));
+        }
+        for (int e = B.e_tail; e < B.e_end; e += LPR) {
+            const int c = col[e];
+            sb = sb + val[e] * (first ? ld3(z, c) : comb(ld3(z, c), ld3(p_old, c)));
+        }
+        sa = group_sum<LPR>(sa);
+        sb = group_sum<LPR>(sb);
+        if (sub == 0) {
+            if (ha) {
+                const float3 pn = first ? zia : comb(zia, pia);
+                const float3 qv = sa + dia * pn;
+                st3(p_new, ia, pn);
+                st3(q, ia, qv);
+                acc[0] += (double)pn.x * qv.x;
+                acc[1] += (double)pn.y * qv.y;
+                acc[2] += (double)pn.z * qv.z;
+            }
+            if (hb) {
+                const float3 pn = first ? zib : comb(zib, pib);
+                const float3 qv = sb + dib * pn;
+                st3(p_new, ib, pn);
+                st3(q, ib, qv);
+                acc[0] += (double)pn.x * qv.x;
+                acc[1] += (double)pn.y * qv.y;
+                acc[2] += (double)pn.z * qv.z;
+            }
         }
     }
+    if (need_be) cg_advance(part, nparts, st, slot_old, be); // a block without rows still takes part
     block_sum_write<3, PBLK>(acc, part, 9, 0);
 }
 
@@ -902,39 +970,85 @@ __global__ void __launch_bounds__(PBLK) k_cg_update_pre(int n, int slot, int npa
                                                        float* __restrict__ d, const float* __restrict__ r_old,
                                                        float* __restrict__ r_new, float* __restrict__ x0, float* __restrict__ res,
                                                        double* __restrict__ part) {
-    double pq[3];
-    block_reduce_parts<3, PBLK>(part, nparts, 9, 0, pq);
-    const double* s = st + slot * 8;
-    float al[3];
-    bool all_done = true;
-    for (int k = 0; k < 3; k++) {
-        al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
-        all_done = all_done && s[3 + k] != 0.0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0 && !all_done) st[19] += 1.0;
     const int sub = threadIdx.x % LPR;
+    const int rpb = PBLK / LPR, stride = gridDim.x * rpb;
+    float al[3] = {0.f, 0.f, 0.f};
+    bool need_al = true;
+    auto take_alpha = [&] {
+        double pq[3];
+        block_reduce_parts<3, PBLK>(part, nparts, 9, 0, pq);
+        const double* s = st + slot * 8;
+        bool all_done = true;
+        for (int k = 0; k < 3; k++) {
+            al[k] = (s[3 + k] != 0.0 || pq[k] == 0.0) ? 0.f : (float)(s[k] / pq[k]);
+            all_done = all_done && s[3 + k] != 0.0;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0 && !all_done) st[19] += 1.0;
+        need_al = false;
+    };
+    auto rnew = [&](float3 rc, float3 qc) { return f3(rc.x - al[0] * qc.x, rc.y - al[1] * qc.y, rc.z - al[2] * qc.z); };
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
-        const int i = r0 + threadIdx.x / LPR;
-        float3 sm = f3(0.f, 0.f, 0.f);
-        row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) {
-            const float3 rc = ld3(r_old, c), qc = ld3(q, c);
-            const float3 rn = f3(rc.x - al[0] * qc.x, rc.y - al[1] * qc.y, rc.z - al[2] * qc.z);
-            sm = sm + (v * (w * dinv[c])) * rn;
-        });
-        sm = group_sum<LPR>(sm);
-        if (i < n && sub == 0) {
-            const float3 ri = ld3(r_old, i), qi = ld3(q, i), pi = ld3(pd, i), di = ld3(d, i);
-            const float3 rn = f3(ri.x - al[0] * qi.x, ri.y - al[1] * qi.y, ri.z - al[2] * qi.z);
-            st3(d, i, f3(di.x + al[0] * pi.x, di.y + al[1] * pi.y, di.z + al[2] * pi.z));
-            st3(r_new, i, rn);
-            st3(x0, i, (w * dinv[i]) * rn);
-            st3(res, i, rn - sm);
-            acc[0] += (double)rn.x * rn.x;
-            acc[1] += (double)rn.y * rn.y;
-            acc[2] += (double)rn.z * rn.z;
+    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+        const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
+        const bool ha = ia < n, hb = ib < n;
+        RowSlot<LPR> A, B;
+        A.load(ia, n, sub, ptr, col, val);
+        B.load(ib, n, sub, ptr, col, val);
+        const float3 ria = ld3_or0(r_old, ia, ha), qia = ld3_or0(q, ia, ha), pia = ld3_or0(pd, ia, ha), dia = ld3_or0(d, ia, ha);
+        const float3 rib = ld3_or0(r_old, ib, hb), qib = ld3_or0(q, ib, hb), pib = ld3_or0(pd, ib, hb), dib = ld3_or0(d, ib, hb);
+        const float wia = ha ? w * dinv[ia] : 0.f, wib = hb ? w * dinv[ib] : 0.f;
+        float3 ra[RU], qa[RU], rb[RU], qb[RU];
+        float wa[RU], wb[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            ra[u] = ld3(r_old, A.c[u]);
+            qa[u] = ld3(q, A.c[u]);
+            wa[u] = dinv[A.c[u]];
+            rb[u] = ld3(r_old, B.c[u]);
+            qb[u] = ld3(q, B.c[u]);
+            wb[u] = dinv[B.c[u]];
+        }
+        if (need_al) take_alpha(); // uniform per block (first row pass)
+        float3 sa = f3(0.f, 0.f, 0.f), sb = f3(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            sa = sa + (A.v[u] * (w * wa[u])) * rnew(ra[u], qa[u]);
+            sb = sb + (B.v[u] * (w * wb[u])) * rnew(rb[u], qb[u]);
+        }
+        for (int e = A.e_tail; e < A.e_end; e += LPR) {
+            const int c = col[e];
+            sa = sa + (val[e] * (w * dinv[c])) * rnew(ld3(r_old, c), ld3(q, c));
+        }
+        for (int e = B.e_tail; e < B.e_end; e += LPR) {
+            const int c = col[e];
+            sb = sb + (val[e] * (w * dinv[c])) * rnew(ld3(r_old, c), ld3(q, c));
+        }
+        sa = group_sum<LPR>(sa);
+        sb = group_sum<LPR>(sb);
+        if (sub == 0) {
+            if (ha) {
+                const float3 rn = rnew(ria, qia);
+                st3(d, ia, f3(dia.x + al[0] * pia.x, dia.y + al[1] * pia.y, dia.z + al[2] * pia.z));
+                st3(r_new, ia, rn);
+                st3(x0, ia, wia * rn);
+                st3(res, ia, rn - sa);
+                acc[0] += (double)rn.x * rn.x;
+                acc[1] += (double)rn.y * rn.y;
+                acc[2] += (double)rn.z * rn.z;
+            }
+            if (hb) {
+                const float3 rn = rnew(rib, qib);
+                st3(d, ib, f3(dib.x + al[0] * pib.x, dib.y + al[1] * pib.y, dib.z + al[2] * pib.z));
+                st3(r_new, ib, rn);
+                st3(x0, ib, wib * rn);
+                st3(res, ib, rn - sb);
+                acc[0] += (double)rn.x * rn.x;
+                acc[1] += (double)rn.y * rn.y;
+                acc[2] += (double)rn.z * rn.z;
+            }
         }
     }
+    if (need_al) take_alpha(); // a block without rows still takes part (block 0 counts the iteration)
     block_sum_write<3, PBLK>(acc, part, 9, 3);
 }
 
@@ -973,24 +1087,55 @@ __global__ void __launch_bounds__(BLK) k_amg_spmv(int rows, const int32_t* __res
 }
 
 // Post-smoothing: xo = xi + w D^-1 (b - A xi); with part != null also b.xo at
-// 6..8 (on the finest level b is the PCG residual, so this is r.z).
+// 6..8 (on the finest level b is the PCG residual, so this is r.z).  Two rows
+// per lane group, as the fused PCG kernels.
 template <int LPR>
 __global__ void __launch_bounds__(PBLK) k_amg_post(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                   const float* __restrict__ val, const float* __restrict__ dinv, float w,
                                                   const float* __restrict__ b, const float* __restrict__ xi, float* __restrict__ xo,
                                                   double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
+    const int rpb = PBLK / LPR, stride = gridDim.x * rpb;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
-        const int i = r0 + threadIdx.x / LPR;
-        const float3 ax = row_dot<LPR>(i, n, sub, ptr, col, val, xi);
-        if (i < n && sub == 0) {
-            const float3 bi = ld3(b, i);
-            const float3 v = ld3(xi, i) + (w * dinv[i]) * (bi - ax);
-            st3(xo, i, v);
-            acc[0] += (double)bi.x * v.x;
-            acc[1] += (double)bi.y * v.y;
-            acc[2] += (double)bi.z * v.z;
+    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+        const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
+        const bool ha = ia < n, hb = ib < n;
+        RowSlot<LPR> A, B;
+        A.load(ia, n, sub, ptr, col, val);
+        B.load(ib, n, sub, ptr, col, val);
+        const float3 bia = ld3_or0(b, ia, ha), xia = ld3_or0(xi, ia, ha), bib = ld3_or0(b, ib, hb), xib = ld3_or0(xi, ib, hb);
+        const float wia = ha ? w * dinv[ia] : 0.f, wib = hb ? w * dinv[ib] : 0.f;
+        float3 xa[RU], xb[RU];
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            xa[u] = ld3(xi, A.c[u]);
+            xb[u] = ld3(xi, B.c[u]);
+        }
+        float3 sa = f3(0.f, 0.f, 0.f), sb = f3(0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < RU; u++) {
+            sa = sa + A.v[u] * xa[u];
+            sb = sb + B.v[u] * xb[u];
+        }
+        for (int e = A.e_tail; e < A.e_end; e += LPR) sa = sa + val[e] * ld3(xi, col[e]);
+        for (int e = B.e_tail; e < B.e_end; e += LPR) sb = sb + val[e] * ld3(xi, col[e]);
+        sa = group_sum<LPR>(sa);
+        sb = group_sum<LPR>(sb);
+        if (sub == 0) {
+            if (ha) {
+                const float3 v = xia + wia * (bia - sa);
+                st3(xo, ia, v);
+                acc[0] += (double)bia.x * v.x;
+                acc[1] += (double)bia.y * v.y;
+                acc[2] += (double)bia.z * v.z;
+            }
+            if (hb) {
+                const float3 v = xib + wib * (bib - sb);
+                st3(xo, ib, v);
+                acc[0] += (double)bib.x * v.x;
+                acc[1] += (double)bib.y * v.y;
+                acc[2] += (double)bib.z * v.z;
+            }
         }
     }
     if (part) block_sum_write<3, PBLK>(acc, part, 9, 6);
@@ -1083,6 +1228,8 @@ Backend::Backend(const pa_device_opts* o) {
 
 Backend::~Backend() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& g : pcg_graph)
+        if (g) (void)hipGraphExecDestroy(g);
     for (auto e : ev_pool) (void)hipEventDestroy(e);
     // an attached external detector goes back to its own stream
     if (det && det != own_det.get()) det->stream = det->own_stream;
@@ -1100,6 +1247,7 @@ void Backend::apply_config(const pa_admm_config& c) {
     PA_CHECK(!initialized, PA_ERR_INVALID, "apply_config must precede init");
     cfg = c;
     if (cfg.residual_interval < 1) cfg.residual_interval = 1;
+    use_graphs = cfg.graphs != 0;
     have_cfg = true;
 }
 
@@ -1539,7 +1687,8 @@ void Backend::assemble_and_upload(const double* x0h) {
     cg_r2.alloc(3 * (size_t)n);
     cg_p2.alloc(3 * (size_t)n);
     a_lpr = lanes_per_row(nnz_a_off + n, n);
-    cg_blocks = std::max(1, std::min(grid_for((int64_t)n * a_lpr, PBLK), CG_MAX_BLOCKS));
+    // two rows per lane group in the fused PCG kernels
+    cg_blocks = std::max(1, std::min(grid_for((int64_t)n * a_lpr, 2 * PBLK), CG_MAX_BLOCKS));
     cg_part.alloc(9 * (size_t)cg_blocks);
     cg_state.alloc(24);
     int elem_blocks = grid_for(hc.n_tri, BLK) + grid_for(hc.n_tet, BLK) + grid_for(hc.n_bend, BLK) + grid_for(hc.n_pin, BLK);
@@ -1565,6 +1714,9 @@ void Backend::attach_detector(Detector* d) {
     PA_CHECK(d, PA_ERR_INVALID, "null detector");
     if (initialized) PA_CHECK(d->n_all == n_all, PA_ERR_INVALID, "detector vertex count differs from the backend");
     PA_CHECK(d->c.cap <= cfg.max_contacts, PA_ERR_INVALID, "detector capacity exceeds max_contacts");
+    // work still queued on the detector's previous stream (the tail of a
+    // detection) must finish before the backend's stream reads its lists
+    if (d->stream && d->stream != stream) PA_HIP(hipStreamSynchronize(d->stream));
     det = d;
     det->set_stream(stream);
 }
@@ -1789,12 +1941,113 @@ void Backend::precondition(const float* r, bool skip_pre0) {
 // launched in chunks between convergence read-backs; the first chunk is sized
 // by the previous solve's count (consecutive ADMM iterations need similar
 // counts).  Returns the PCG iterations summed over passes.
+// One PCG iteration `it` (state slot it & 1).  Fused AMG path: k_cg_dir_spmv,
+// k_cg_update_pre and the rest of the V-cycle, with the p and r ping-pong
+// buffers selected by pc / rc (toggled here).  Plain path: SpMV, update,
+// preconditioner, direction.
+void Backend::pcg_iteration(int it, bool fused, int& pc, int& rc) {
+    const int nb = cg_blocks;
+    const int slot = it & 1;
+    float* pbuf[2] = {cg_p.p, cg_p2.p};
+    float* rbuf[2] = {cg_r.p, cg_r2.p};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const bool mark = (profiling & 1) && !capturing;
+    if (mark) {
+        e0 = next_event();
+        e1 = next_event();
+        PA_HIP(hipEventRecord(e0, stream));
+    }
+    if (fused) {
+        with_lpr(a_lpr, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, it == 0 ? 1 : 0, slot ^ 1, nb, cg_state.p, a_ptr.p,
+                               a_col.p, a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)pbuf[pc], pbuf[pc ^ 1], cg_q.p,
+                               cg_part.p);
+        });
+        pc ^= 1;
+    } else {
+        with_lpr(a_lpr, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
+                               (const float*)cg_p.p, cg_q.p, cg_part.p);
+        });
+    }
+    PA_LAUNCH_CHECK();
+    if (mark) {
+        PA_HIP(hipEventRecord(e1, stream));
+        spmv_marks.push_back({e0, e1});
+    }
+    if (fused) {
+        DevLevel& L0 = *amg[0];
+        with_lpr(L0.lpr_a, [&](auto c) {
+            constexpr int P = decltype(c)::value;
+            hipLaunchKernelGGL(k_cg_update_pre<P>, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, L0.aptr.p, L0.acol.p,
+                               L0.aval.p, L0.dinv.p, L0.omega, (const float*)pbuf[pc], (const float*)cg_q.p, cg_d.p,
+                               (const float*)rbuf[rc], rbuf[rc ^ 1], L0.x.p, L0.r.p, cg_part.p);
+        });
+        PA_LAUNCH_CHECK();
+        rc ^= 1;
+        precondition(rbuf[rc], true);
+    } else {
+        hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_p.p, cg_q.p, cg_d.p, cg_r.p,
+                           cg_part.p);
+        PA_LAUNCH_CHECK();
+        precondition(cg_r.p);
+        hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
+        PA_LAUNCH_CHECK();
+    }
+}
+
+// Two fused iterations (it, it + 1 with it even, so the state slots and the
+// ping-pong buffers return to where they started) captured once as a HIP
+// graph: index 1 holds the first pair of a solve (it = 0, direction = z),
+// index 0 every later pair.  One graph launch replaces twenty kernel
+// launches, which the host cannot issue as fast as these short kernels run.
+// The captured arguments are the backend's persistent buffers, so the graphs
+// stay valid for the backend's lifetime.
+hipGraphExec_t Backend::pcg_pair_graph(bool first_pair) {
+    hipGraphExec_t& g = pcg_graph[first_pair ? 1 : 0];
+    if (g) return g;
+    hipGraph_t graph = nullptr;
+    PA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    capturing = true;
+    try {
+        int pc = 0, rc = 0;
+        pcg_iteration(first_pair ? 0 : 2, true, pc, rc);
+        pcg_iteration(first_pair ? 1 : 3, true, pc, rc);
+    } catch (...) {
+        capturing = false;
+        hipGraph_t tmp = nullptr;
+        (void)hipStreamEndCapture(stream, &tmp);
+        if (tmp) (void)hipGraphDestroy(tmp);
+        throw;
+    }
+    capturing = false;
+    PA_HIP(hipStreamEndCapture(stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        g = nullptr;
+        PA_HIP(e);
+    }
+    return g;
+}
+
+// Mixed-precision refinement: each pass computes the FP64 residual
+// r_0 = b - A x of the FP32 iterate, solves A d = r_0 with preconditioned FP32
+// CG until |r_k| <= tol |r_0| per column (the recursive residual), and adds d.
+// A later pass runs only while the FP64 residual exceeds 1e-12 |b|.
+// Iterations are launched in chunks between convergence read-backs (the first
+// chunk sized by the previous solve's count); on the fused path a chunk is a
+// sequence of graph-launched iteration pairs.  Returns the PCG iterations
+// summed over passes.
 int Backend::global_solve(int max_iter, double tol) {
     const int nb = cg_blocks;
     const int passes = std::max(1, cfg.global_refine);
     // the fused iteration needs a real level 0 (nl >= 2); a one-level
     // hierarchy is an exact coarse solve and keeps the plain sequence
     const bool fused = use_amg && amg.size() >= 2;
+    const bool graphs = fused && use_graphs && !(profiling & 1);
     int total_iters = 0;
     double hs[24];
     for (int pass = 0; pass < passes; pass++) {
@@ -1816,60 +2069,20 @@ int Backend::global_solve(int max_iter, double tol) {
             PA_HIP(hipStreamSynchronize(stream));
             if (hs[3] != 0 && hs[4] != 0 && hs[5] != 0) break;
         }
-        float* pbuf[2] = {cg_p.p, cg_p2.p};
-        float* rbuf[2] = {cg_r.p, cg_r2.p};
-        int pc = 0, rc_ = 0;
+        int pc = 0, rc = 0;
         int chunk = std::max(2, pass == 0 ? last_pcg_iters : 2);
+        if (graphs) chunk += chunk & 1;
         int it = 0;
         while (it < max_iter) {
             const int stop = std::min(max_iter, it + chunk);
             chunk = 2;
-            for (; it < stop; it++) {
-                const int slot = it & 1;
-                hipEvent_t e0 = nullptr, e1 = nullptr;
-                if (profiling & 1) {
-                    e0 = next_event();
-                    e1 = next_event();
-                    PA_HIP(hipEventRecord(e0, stream));
-                }
-                if (fused) {
-                    with_lpr(a_lpr, [&](auto c) {
-                        constexpr int P = decltype(c)::value;
-                        hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, it == 0 ? 1 : 0, slot ^ 1, nb, cg_state.p,
-                                           a_ptr.p, a_col.p, a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)pbuf[pc],
-                                           pbuf[pc ^ 1], cg_q.p, cg_part.p);
-                    });
-                    pc ^= 1;
+            while (it < stop) {
+                if (graphs && it + 2 <= stop && (it & 1) == 0 && pc == 0 && rc == 0) {
+                    PA_HIP(hipGraphLaunch(pcg_pair_graph(it == 0), stream));
+                    it += 2;
                 } else {
-                    with_lpr(a_lpr, [&](auto c) {
-                        constexpr int P = decltype(c)::value;
-                        hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
-                                           (const float*)cg_p.p, cg_q.p, cg_part.p);
-                    });
-                }
-                PA_LAUNCH_CHECK();
-                if (profiling & 1) {
-                    PA_HIP(hipEventRecord(e1, stream));
-                    spmv_marks.push_back({e0, e1});
-                }
-                if (fused) {
-                    DevLevel& L0 = *amg[0];
-                    with_lpr(L0.lpr_a, [&](auto c) {
-                        constexpr int P = decltype(c)::value;
-                        hipLaunchKernelGGL(k_cg_update_pre<P>, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, L0.aptr.p,
-                                           L0.acol.p, L0.aval.p, L0.dinv.p, L0.omega, (const float*)pbuf[pc], (const float*)cg_q.p,
-                                           cg_d.p, (const float*)rbuf[rc_], rbuf[rc_ ^ 1], L0.x.p, L0.r.p, cg_part.p);
-                    });
-                    PA_LAUNCH_CHECK();
-                    rc_ ^= 1;
-                    precondition(rbuf[rc_], true);
-                } else {
-                    hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_p.p, cg_q.p, cg_d.p,
-                                       cg_r.p, cg_part.p);
-                    PA_LAUNCH_CHECK();
-                    precondition(cg_r.p);
-                    hipLaunchKernelGGL(k_cg_dir, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, cg_z.p, cg_p.p, cg_part.p);
-                    PA_LAUNCH_CHECK();
+                    pcg_iteration(it, fused, pc, rc);
+                    it++;
                 }
             }
             if (fused) { // the state slot of iteration `it`, which k_cg_dir_spmv would compute next

@@ -256,6 +256,33 @@ def test_step_with_rotating_obstacle_matches_oracle(pa):
     assert np.abs(vg[n:]).max() > 0.1  # the obstacle moves
 
 
+@pytest.mark.parametrize("name,kind", [("tetbox", 2), ("cloth256", 3)])
+def test_graph_replay_matches_eager(pa, name, kind):
+    """PCG iteration pairs replayed as HIP graphs (the default) against the same
+    kernels launched one by one: every step's state is bitwise identical.  Each
+    step replays the graphs many times with eager work (detection, local and
+    contact steps, read-backs) between the replays."""
+    from parallel_admm_amd.configs import config_for
+
+    sc = pa.Scene.generate(kind)
+    states = {}
+    for graphs in (0, 1):
+        cfg = config_for(name)
+        cfg.graphs = graphs
+        cfg.admm_max_iter = 12
+        b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+        if name == "tetbox":
+            b.set_state(x=drape_on_obstacle(sc, cfg.thickness, frac=2.0))
+        out = []
+        for _ in range(3):
+            st = b.step()
+            out.append((b.get_state()[0], st["global_iters"]))
+        states[graphs] = out
+    for (xe, ie), (xg, ig) in zip(states[0], states[1]):
+        assert ie == ig and ie > 0
+        np.testing.assert_array_equal(xg, xe)
+
+
 def test_step_deterministic(pa):
     """Two runs from the same state are bitwise identical."""
     from parallel_admm_amd.configs import config_for
