@@ -212,6 +212,10 @@ typedef struct pa_step_stats {
     int32_t detections;
     double r_primal, r_dual; /* last residual check */
     double step_ms;          /* host wall time of the step */
+    int32_t ws_seeded;       /* detections whose contact duals were warm-started (kept attempt) */
+    int32_t ws_fallback;     /* 1: the warm-started attempt was abandoned and the step re-run cold */
+    int32_t ws_wasted_iters; /* ADMM iterations of the abandoned attempt */
+    int32_t ws_bad_windows;  /* seeded windows judged bad (see pa_warm_start.patience) */
 } pa_step_stats;
 
 pa_status pa_backend_create(const pa_device_opts* opts, pa_backend** out);
@@ -296,6 +300,85 @@ pa_status pa_backend_get_sizes(pa_backend* b, int64_t* out7);
  * on the current device state between two events and returns the average per
  * launch (ms); call between steps (it writes only PCG scratch). */
 pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, double* avg_ms, int64_t* launches);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start of the contact duals.  No counterpart in the reference:
+ * there the contact duals r_c restart from zero at every detection
+ * (src/solver/contact_solver.cu:243; CPU src/contact/contact_info.h:46).
+ * At each detection a small MLP (10 -> 128 -> 128 -> 128 -> 2, SiLU, BF16
+ * weights, FP32 accumulation on MFMA) maps per-contact features to an initial
+ * dual; plain cold start stays as the fallback.
+ *
+ * Features of a contact at the start of its detection window (raw units; h =
+ * time step, dhat = contact distance, s = h / dhat, n = contact normal, p0 =
+ * (x - x0 + Uc) / h on dynamic and the static velocity on static vertices,
+ * u0 = sum_k b_k p0_k + K_c, v = velocities at the step start, t = unit
+ * tangential part of u0 (any unit vector normal to n when it vanishes),
+ * lambda_prev = the dual of the same contact (same type and primitives) at
+ * the end of the previous window):
+ *   0 h_cN / dhat          1 s (u0.n)              2 s |u0 - (u0.n) n|
+ *   3 s (v_rel.n)          4 s |v_rel - (v_rel.n) n|  (v_rel = sum_k b_k v_k)
+ *   5 mu                   6 type (0 PT, 1 EE)      7 1 if lambda_prev exists
+ *   8 s (lambda_prev.n)    9 s (lambda_prev.t)
+ * Outputs: s (lambda.n), s (lambda.t); the seeded dual is lambda.
+ * ------------------------------------------------------------------------- */
+#define PA_MLP_FEATURES 10
+#define PA_MLP_OUTPUTS 2
+#define PA_MLP_PARAMS 34690 /* W1[128][10] b1[128] W2[128][128] b2 W3[128][128] b3 W4[2][128] b4[2] */
+
+typedef struct pa_warm_start {
+    int32_t mode;        /* 0 cold start, 1 learned MLP, 2 the previous window's dual of the same contact */
+    int32_t patience;    /* seeded windows in a row judged bad, sum |r_final - seed|^2 > 4 sum |r_final|^2
+                            (the seeds miss the window's final duals by more than twice what zero would),
+                            before the step is re-run cold; 0 = 2 */
+    double stall_ratio;  /* a residual check above stall_ratio x the step's best so far, or non-finite,
+                            re-runs the step cold; 0 = 1000 (the residual jumps by 10x and more after a
+                            detection brings new contacts, in plain steps too) */
+    double clamp;        /* |seed component| <= clamp in units of dhat / h; 0 = 50 */
+    const float* params; /* mode 1: PA_MLP_PARAMS weights in the flat layout above (copied) */
+    float in_shift[PA_MLP_FEATURES], in_scale[PA_MLP_FEATURES]; /* network input = (feature - shift) * scale */
+    float out_scale[PA_MLP_OUTPUTS];                              /* s lambda = network output * out_scale */
+} pa_warm_start;
+
+/* Re-running cold replays the step from its starting state, so a fallback step
+ * is bit-identical to a plain step. */
+pa_status pa_backend_set_warm_start(pa_backend* b, const pa_warm_start* ws);
+/* Contact-state log (training data): at the end of each detection window one
+ * row of 12 floats per contact, the 10 features at the window's start and the
+ * window's final dual as s (lambda.n), s (lambda.t).  capacity 0 disables;
+ * rows beyond the capacity are dropped. */
+pa_status pa_backend_set_contact_log(pa_backend* b, int64_t capacity_rows);
+/* Copies up to cap_rows rows and empties the log; *rows = rows copied. */
+pa_status pa_backend_read_contact_log(pa_backend* b, float* out, int64_t cap_rows, int64_t* rows);
+/* Seed log of the last step: per detection of the kept attempt, the 3 n_c
+ * seeded duals in contact order (counts[k] = 0 when detection k was not seeded). */
+pa_status pa_backend_set_seed_log(pa_backend* b, int32_t enabled);
+pa_status pa_backend_read_seed_log(pa_backend* b, float* out, int64_t cap_floats, int32_t* counts, int32_t max_detections,
+                                   int32_t* n_detections);
+/* Current contact duals r_c (3 n_contacts, contact order). */
+pa_status pa_backend_get_contact_duals(pa_backend* b, double* rc);
+
+/* MLP handle for training on the device (forward, backward and Adam on MFMA). */
+typedef struct pa_mlp pa_mlp;
+/* Weights and biases ~ U(-1/sqrt(fan_in), 1/sqrt(fan_in)) from `seed`. */
+pa_status pa_mlp_create(uint64_t seed, pa_mlp** out);
+void pa_mlp_destroy(pa_mlp* m);
+pa_status pa_mlp_get_params(pa_mlp* m, float* params);
+/* Also resets the Adam state. */
+pa_status pa_mlp_set_params(pa_mlp* m, const float* params);
+/* y[n][2] = MLP(x[n][10]). */
+pa_status pa_mlp_forward(pa_mlp* m, const float* x, int32_t n, float* y);
+/* Gradient of L = 1/(2n) sum |MLP(x) - t|^2 (t[n][2]) in the flat layout; *loss = L. */
+pa_status pa_mlp_gradients(pa_mlp* m, const float* x, const float* t, int32_t n, float* grad, double* loss);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on that loss; *loss = L before the step. */
+pa_status pa_mlp_adam_step(pa_mlp* m, const float* x, const float* t, int32_t n, double lr, double* loss);
+/* Adam over `epochs` shuffled passes (seeded) of samples[n][12] (10 inputs, 2
+ * targets) in batches of `batch` (a multiple of 32); epoch_loss[e] = mean L. */
+pa_status pa_mlp_train(pa_mlp* m, const float* samples, int64_t n, int32_t epochs, int32_t batch, double lr, uint64_t seed,
+                       double* epoch_loss);
+/* Average ms per forward (which 0) or forward + backward + Adam (which 1) of
+ * `rows` rows over `reps` back-to-back runs (weights kept, Adam state reset). */
+pa_status pa_mlp_time(pa_mlp* m, int32_t which, int32_t rows, int32_t reps, double* ms);
 
 /* Message of the last failed call on this host thread (any handle). */
 const char* pa_last_error_message(void);

@@ -1241,6 +1241,10 @@ template <class R> struct Solver {
     std::vector<R> r_c;
     std::vector<double> residual_log; // r_p, r_d per check
     bool brute_force = false;
+    // Seeds of the contact duals for the next step only: entry k (3 n_c values,
+    // contact order) replaces the zero start of r_c at the step's k-th
+    // detection when its size matches; other detections start from zero.
+    std::vector<std::vector<double>> seed_schedule;
     // Rigid tracks of static vertex ranges (ScriptAnimator rigid scripts,
     // src/solver/animator.cpp:9-39, 83-130; see add_rigid_track).
     struct Track {
@@ -1557,11 +1561,14 @@ template <class R> struct Solver {
         if (!cfg.warmstart_uc) std::fill(Uc.begin(), Uc.end(), R(0));
         std::vector<R> z(3 * m, R(0)), zprev(3 * m, R(0)), DX, t1(3 * n), b(3 * n), tmp;
         bool have_contacts = false;
-        int it = 0;
+        int it = 0, ndet = 0;
         for (; it < cfg.admm_max_iter; it++) {
             if (cfg.enable_contact && (it % cfg.dcd_interval == 0)) {
                 contacts = detect<R>(T, xc.data(), R(cfg.thickness), brute_force);
                 compute_scc(eps, gam, hinv);
+                if (ndet < (int)seed_schedule.size() && seed_schedule[ndet].size() == r_c.size())
+                    for (size_t q = 0; q < r_c.size(); q++) r_c[q] = R(seed_schedule[ndet][q]);
+                ndet++;
                 have_contacts = true;
             }
             zprev = z;
@@ -1613,6 +1620,7 @@ template <class R> struct Solver {
                 }
             }
         }
+        seed_schedule.clear();
         st.iterations = it;
         st.n_contacts = (int)contacts.size();
         for (int i = 0; i < 3 * n; i++) v[i] = (xc[i] - x0[i]) / h;

@@ -146,6 +146,20 @@ int orc_add_rigid_track(void* hp, int first, int count, int frames, const double
     return 0;
 }
 
+// Seeds of the contact duals at each detection of the next step (see
+// Solver::seed_schedule): counts[k] values for detection k, concatenated.
+void orc_set_seed_schedule(void* hp, const double* seeds, const int* counts, int ndet) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, {
+        s.seed_schedule.assign(ndet, {});
+        size_t o = 0;
+        for (int k = 0; k < ndet; k++) {
+            s.seed_schedule[k].assign(seeds + o, seeds + o + counts[k]);
+            o += counts[k];
+        }
+    });
+}
+
 // sizes: n_all, n_dyn, m, nnz_D, nnz_A, n_surf_v, n_surf_t, n_surf_e, n_constraints, n_contacts, nnz_L
 void orc_sizes(void* hp, long long* out) {
     auto* h = static_cast<Handle*>(hp);

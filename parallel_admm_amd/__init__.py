@@ -37,8 +37,14 @@ EXPORTS = [
     "pa_backend_reset", "pa_backend_get_state", "pa_backend_set_state", "pa_backend_get_duals",
     "pa_backend_set_duals", "pa_backend_set_static_state", "pa_backend_create_from_scene",
     "pa_backend_project_elastic", "pa_backend_project_contacts", "pa_backend_global_solve",
-    "pa_backend_add_rigid_track", "pa_backend_set_profiling", "pa_backend_get_sizes", "pa_backend_phase_times", "pa_backend_kernel_time", "pa_last_error_message",
+    "pa_backend_add_rigid_track", "pa_backend_set_profiling", "pa_backend_get_sizes", "pa_backend_phase_times", "pa_backend_kernel_time",
+    "pa_backend_set_warm_start", "pa_backend_set_contact_log", "pa_backend_read_contact_log", "pa_backend_set_seed_log",
+    "pa_backend_read_seed_log", "pa_backend_get_contact_duals", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_get_params",
+    "pa_mlp_set_params", "pa_mlp_forward", "pa_mlp_gradients", "pa_mlp_adam_step", "pa_mlp_train", "pa_mlp_time",
+    "pa_last_error_message",
 ]
+
+MLP_FEATURES, MLP_OUTPUTS, MLP_PARAMS = 10, 2, 34690
 
 
 class PAError(RuntimeError):
@@ -93,7 +99,13 @@ class DeviceOptsC(C.Structure):
 class StepStatsC(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("n_contacts", C.c_int32), ("converged", C.c_int32), ("overflow", C.c_int32),
                 ("global_iters", C.c_int32), ("detections", C.c_int32), ("r_primal", C.c_double), ("r_dual", C.c_double),
-                ("step_ms", C.c_double)]
+                ("step_ms", C.c_double), ("ws_seeded", C.c_int32), ("ws_fallback", C.c_int32), ("ws_wasted_iters", C.c_int32),
+                ("ws_bad_windows", C.c_int32)]
+
+
+class WarmStartC(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("patience", C.c_int32), ("stall_ratio", C.c_double), ("clamp", C.c_double),
+                ("params", C.c_void_p), ("in_shift", C.c_float * 10), ("in_scale", C.c_float * 10), ("out_scale", C.c_float * 2)]
 
 
 _lib: Optional[C.CDLL] = None
@@ -111,6 +123,12 @@ def lib() -> C.CDLL:
         L.pa_backend_last_error.restype = C.c_char_p
         L.pa_last_error_message.restype = C.c_char_p
         L.pa_scene_generate.argtypes = [C.c_int32, C.c_int32, C.c_uint64, C.c_double, C.c_double, C.c_void_p]
+        L.pa_backend_set_contact_log.argtypes = [C.c_void_p, C.c_int64]
+        L.pa_backend_read_contact_log.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p]
+        L.pa_backend_read_seed_log.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_int32, C.c_void_p]
+        L.pa_mlp_create.argtypes = [C.c_uint64, C.c_void_p]
+        L.pa_mlp_adam_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_double, C.c_void_p]
+        L.pa_mlp_train.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_uint64, C.c_void_p]
         _lib = L
     return _lib
 
@@ -487,6 +505,117 @@ class ADMMBackendHIP:
         n = C.c_int64(int(reps))
         self._err(lib().pa_backend_kernel_time(self._h, 1, 0, C.byref(avg), C.byref(n)), "spmv_time")
         return avg.value
+
+
+    # --- learned warm start of the contact duals ---
+    def set_warm_start(self, mode: int, params=None, in_shift=None, in_scale=None, out_scale=None, patience: int = 0,
+                       stall_ratio: float = 0.0, clamp: float = 0.0) -> None:
+        """mode 0 cold start, 1 learned MLP (params: MLP_PARAMS floats), 2 the
+        previous window's dual of the same contact (pa_warm_start)."""
+        w = WarmStartC()
+        w.mode, w.patience, w.stall_ratio, w.clamp = int(mode), int(patience), float(stall_ratio), float(clamp)
+        keep = None
+        if params is not None:
+            keep = np.ascontiguousarray(params, np.float32).reshape(-1)
+            if keep.size != MLP_PARAMS:
+                raise ValueError(f"expected {MLP_PARAMS} parameters, got {keep.size}")
+            w.params = keep.ctypes.data
+        for name, val, n, default in (("in_shift", in_shift, 10, 0.0), ("in_scale", in_scale, 10, 1.0), ("out_scale", out_scale, 2, 1.0)):
+            a = np.full(n, default, np.float32) if val is None else np.asarray(val, np.float32).reshape(n)
+            getattr(w, name)[:] = a.tolist()
+        self._err(lib().pa_backend_set_warm_start(self._h, C.byref(w)), "set_warm_start")
+
+    def set_contact_log(self, capacity_rows: int) -> None:
+        self._err(lib().pa_backend_set_contact_log(self._h, int(capacity_rows)), "set_contact_log")
+
+    def read_contact_log(self, max_rows: int) -> np.ndarray:
+        """Rows of 12 floats (10 features, 2 final-dual targets); empties the log."""
+        out = np.zeros((max(int(max_rows), 1), 12), np.float32)
+        rows = C.c_int64()
+        self._err(lib().pa_backend_read_contact_log(self._h, _ptr(out), int(max_rows), C.byref(rows)), "read_contact_log")
+        return out[:rows.value]
+
+    def set_seed_log(self, enabled: bool) -> None:
+        self._err(lib().pa_backend_set_seed_log(self._h, int(enabled)), "set_seed_log")
+
+    def read_seed_log(self, max_floats: int = 1 << 24, max_detections: int = 1024) -> list:
+        """Seeds of the last step per detection: arrays (n_c, 3), or None where not seeded."""
+        out = np.zeros(max(int(max_floats), 1), np.float32)
+        counts = np.zeros(max_detections, np.int32)
+        nd = C.c_int32()
+        self._err(lib().pa_backend_read_seed_log(self._h, _ptr(out), int(max_floats), _ptr(counts), int(max_detections),
+                                                 C.byref(nd)), "read_seed_log")
+        res, o = [], 0
+        for k in range(nd.value):
+            c = int(counts[k])
+            res.append(out[o:o + c].reshape(-1, 3).astype(np.float64) if c else None)
+            o += c
+        return res
+
+    def contact_duals(self) -> np.ndarray:
+        n = self.sizes()["n_contacts"]
+        rc = np.zeros((max(n, 1), 3))
+        self._err(lib().pa_backend_get_contact_duals(self._h, _ptr(rc)), "get_contact_duals")
+        return rc[:n]
+
+
+class MLP:
+    """The warm-start network (10 -> 128 -> 128 -> 128 -> 2, SiLU) with forward,
+    backward and Adam in BF16-MFMA / FP32-accumulate kernels on the GPU."""
+
+    def __init__(self, seed: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().pa_mlp_create(C.c_uint64(seed), C.byref(self._h)), "mlp create")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pa_mlp_destroy(self._h)
+            self._h = None
+
+    def params(self) -> np.ndarray:
+        w = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().pa_mlp_get_params(self._h, _ptr(w)), "mlp params")
+        return w
+
+    def set_params(self, w) -> None:
+        w = np.ascontiguousarray(w, np.float32).reshape(-1)
+        if w.size != MLP_PARAMS:
+            raise ValueError(f"expected {MLP_PARAMS} parameters")
+        _check(lib().pa_mlp_set_params(self._h, _ptr(w)), "mlp set params")
+
+    def forward(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_FEATURES)
+        y = np.zeros((x.shape[0], MLP_OUTPUTS), np.float32)
+        _check(lib().pa_mlp_forward(self._h, _ptr(x), x.shape[0], _ptr(y)), "mlp forward")
+        return y
+
+    def gradients(self, x, t) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_OUTPUTS)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = C.c_double()
+        _check(lib().pa_mlp_gradients(self._h, _ptr(x), _ptr(t), x.shape[0], _ptr(g), C.byref(loss)), "mlp gradients")
+        return g, loss.value
+
+    def adam_step(self, x, t, lr: float) -> float:
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, MLP_FEATURES)
+        t = np.ascontiguousarray(t, np.float32).reshape(-1, MLP_OUTPUTS)
+        loss = C.c_double()
+        _check(lib().pa_mlp_adam_step(self._h, _ptr(x), _ptr(t), x.shape[0], float(lr), C.byref(loss)), "mlp adam")
+        return loss.value
+
+    def train(self, samples, epochs: int, batch: int = 8192, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
+        """Adam on samples[n][12] (10 normalised inputs, 2 scaled targets); returns the per-epoch mean loss."""
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 12)
+        losses = np.zeros(epochs)
+        _check(lib().pa_mlp_train(self._h, _ptr(s), s.shape[0], int(epochs), int(batch), float(lr), C.c_uint64(seed),
+                                  _ptr(losses)), "mlp train")
+        return losses
+
+    def time_ms(self, which: int, rows: int, reps: int = 100) -> float:
+        ms = C.c_double()
+        _check(lib().pa_mlp_time(self._h, int(which), int(rows), int(reps), C.byref(ms)), "mlp time")
+        return ms.value
 
 
 def gpu_available() -> bool:

@@ -26,6 +26,7 @@ SOURCES = [
     ("pa_prims.hip", True, []),
     ("pa_lbvh.hip", True, ["-ffp-contract=off"]),
     ("pa_solver.hip", True, []),
+    ("pa_mlp.hip", True, []),
 ]
 
 

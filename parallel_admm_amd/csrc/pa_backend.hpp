@@ -12,6 +12,7 @@
 #include "pa_detector.hpp"
 #include "pa_prims.hpp"
 #include "pa_amg.hpp"
+#include "pa_mlp.hpp"
 
 namespace pa {
 
@@ -103,6 +104,31 @@ struct Backend {
     DBuf<float4> gam, kc, rc, du;
     DBuf<uint32_t> vcnt, vptr, vfill, vent;
     PrimScratch scratch;
+    // warm start of the contact duals at each detection (pa_warm_start)
+    pa_warm_start ws{};             // params pointer cleared; weights live in ws_mlp
+    std::unique_ptr<Mlp> ws_mlp;
+    bool ws_active = false;         // seeding in the current attempt of the step
+    bool ws_restart = false;        // the judge or the stall check abandoned the attempt
+    int ws_bad = 0;                 // consecutive windows judged worse than a zero seed
+    int ws_nc = 0;                  // contacts of the current window
+    bool ws_window = false;         // the current window has features (and maybe seeds)
+    bool ws_window_seeded = false;
+    bool ws_judged = false;         // the current window's seeds were judged
+    int ws_seeded_count = 0;
+    int ws_bad_windows = 0;         // windows judged bad during the step (all attempts)
+    DBuf<float> ws_feat, ws_norm;   // 10 raw / normalised features per contact
+    DBuf<float4> ws_tan, ws_seed;   // tangent of the feature frame; seeded dual
+    DBuf<int32_t> ws_match;         // index of the same contact in the previous window, or -1
+    DBuf<int32_t> prev_type, prev_p0, prev_p1;
+    DBuf<float4> prev_rc;
+    int nc_prev = 0;
+    DBuf<double> ws_err;            // judge partial sums
+    DBuf<float> ws_consts;          // in_shift[10], in_scale[10], out_scale[2]
+    DBuf<float> snap_x, snap_v, snap_ue, snap_uc; // step-start state for a cold re-run
+    int64_t log_cap = 0, log_rows = 0;
+    DBuf<float> log_buf;
+    bool seed_log = false;
+    std::vector<std::vector<float>> seed_log_h;
     // residual partial sums
     DBuf<double> res_part, res_sum;
     int res_blocks = 0;
@@ -156,6 +182,17 @@ struct Backend {
     void precondition(const float* r, bool skip_pre0 = false);
     void dual_update(bool want_residual);
     void residual_read(double* rp, double* rd);
+    // warm start: configuration, window end (judge, log, keep the duals for
+    // matching) and window start (features, seeds); see pa_warm_start
+    void set_warm_start(const pa_warm_start& w);
+    void ws_judge();
+    void ws_window_end();
+    void ws_window_begin();
+    void ws_forget() {
+        ws_window = false;
+        nc_prev = 0;
+    }
+    int admm_loop(pa_step_stats& S, float eps, float gam, float hinv);
 
   private:
     void assemble_and_upload(const double* x0);

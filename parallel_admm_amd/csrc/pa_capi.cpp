@@ -7,6 +7,7 @@
 #include "pa_admm.h"
 #include "pa_backend.hpp"
 #include "pa_detector.hpp"
+#include "pa_mlp.hpp"
 #include "pa_scene.hpp"
 
 struct pa_scene {
@@ -19,6 +20,9 @@ struct pa_detector {
 struct pa_backend {
     std::unique_ptr<pa::Backend> b;
     std::string err;
+};
+struct pa_mlp {
+    std::unique_ptr<pa::Mlp> m;
 };
 
 namespace {
@@ -467,6 +471,7 @@ pa_status pa_backend_project_contacts(pa_backend* b, double* p, double* rco) {
         pa::Backend& B = *b->b;
         PA_CHECK(B.initialized && B.det, PA_ERR_INVALID, "init and attach a detector first");
         PA_CHECK(p, PA_ERR_INVALID, "null p");
+        B.ws_forget(); // the detection window of the step loop is replaced
         const double hd = B.cfg.dt;
         const float eps = (float)(1.0 / (hd * hd * B.cfg.kappa + hd * B.cfg.beta));
         const float gm = (float)(hd * B.cfg.kappa / (hd * B.cfg.kappa + B.cfg.beta));
@@ -540,6 +545,112 @@ pa_status pa_backend_kernel_time(pa_backend* b, int32_t which, int32_t reset, do
             B.spmv_launches = 0;
         }
     });
+}
+
+pa_status pa_backend_set_warm_start(pa_backend* b, const pa_warm_start* ws) {
+    BK(PA_CHECK(ws, PA_ERR_INVALID, "null warm start"); b->b->set_warm_start(*ws));
+}
+
+pa_status pa_backend_set_contact_log(pa_backend* b, int64_t cap) {
+    BK({
+        PA_CHECK(cap >= 0 && cap <= ((int64_t)1 << 28), PA_ERR_INVALID, "log capacity out of range");
+        pa::Backend& B = *b->b;
+        B.log_cap = cap;
+        B.log_rows = 0;
+        if (cap) B.log_buf.alloc((size_t)cap * 12);
+        else B.log_buf.release();
+    });
+}
+
+pa_status pa_backend_read_contact_log(pa_backend* b, float* out, int64_t cap_rows, int64_t* rows) {
+    BK({
+        PA_CHECK(rows && (out || cap_rows == 0) && cap_rows >= 0, PA_ERR_INVALID, "null argument");
+        pa::Backend& B = *b->b;
+        const int64_t r = std::min(cap_rows, B.log_rows);
+        if (r) B.log_buf.download(out, (size_t)r * 12, B.stream);
+        PA_HIP(hipStreamSynchronize(B.stream));
+        *rows = r;
+        B.log_rows = 0;
+    });
+}
+
+pa_status pa_backend_set_seed_log(pa_backend* b, int32_t on) {
+    BK(b->b->seed_log = on != 0; b->b->seed_log_h.clear());
+}
+
+pa_status pa_backend_read_seed_log(pa_backend* b, float* out, int64_t cap, int32_t* counts, int32_t max_det, int32_t* ndet) {
+    BK({
+        PA_CHECK(counts && ndet && max_det >= 0, PA_ERR_INVALID, "null argument");
+        const auto& L = b->b->seed_log_h;
+        PA_CHECK((int64_t)L.size() <= max_det, PA_ERR_INVALID, "more detections than max_detections");
+        int64_t total = 0;
+        for (const auto& v : L) total += (int64_t)v.size();
+        PA_CHECK(total <= cap && (out || total == 0), PA_ERR_INVALID, "seed log larger than the output");
+        int64_t o = 0;
+        for (size_t k = 0; k < L.size(); k++) {
+            counts[k] = (int32_t)L[k].size();
+            std::copy(L[k].begin(), L[k].end(), out + o);
+            o += (int64_t)L[k].size();
+        }
+        *ndet = (int32_t)L.size();
+    });
+}
+
+pa_status pa_backend_get_contact_duals(pa_backend* b, double* rco) {
+    BK({
+        pa::Backend& B = *b->b;
+        PA_CHECK(rco || B.nc == 0, PA_ERR_INVALID, "null argument");
+        std::vector<float4> r(B.nc);
+        if (B.nc) B.rc.download(r.data(), B.nc, B.stream);
+        PA_HIP(hipStreamSynchronize(B.stream));
+        for (int c = 0; c < B.nc; c++) {
+            rco[3 * c] = r[c].x;
+            rco[3 * c + 1] = r[c].y;
+            rco[3 * c + 2] = r[c].z;
+        }
+    });
+}
+
+#define MK(...) return guard(nullptr, [&] { PA_CHECK(m && m->m, PA_ERR_INVALID, "null mlp"); __VA_ARGS__; })
+
+pa_status pa_mlp_create(uint64_t seed, pa_mlp** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(out, PA_ERR_INVALID, "null argument");
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw pa::Error(PA_ERR_NO_DEVICE, "no HIP device");
+        auto h = std::make_unique<pa_mlp>();
+        h->m = std::make_unique<pa::Mlp>();
+        h->m->init_params(seed);
+        *out = h.release();
+    });
+}
+void pa_mlp_destroy(pa_mlp* m) { delete m; }
+pa_status pa_mlp_get_params(pa_mlp* m, float* w) { MK(PA_CHECK(w, PA_ERR_INVALID, "null argument"); m->m->get_params(w)); }
+pa_status pa_mlp_set_params(pa_mlp* m, const float* w) { MK(PA_CHECK(w, PA_ERR_INVALID, "null argument"); m->m->set_params(w)); }
+pa_status pa_mlp_forward(pa_mlp* m, const float* x, int32_t n, float* y) {
+    MK(PA_CHECK(x && y && n > 0 && n <= (1 << 24), PA_ERR_INVALID, "bad argument"); m->m->forward_host(x, n, y));
+}
+pa_status pa_mlp_gradients(pa_mlp* m, const float* x, const float* t, int32_t n, float* g, double* loss) {
+    MK({
+        PA_CHECK(x && t && g && n > 0 && n <= (1 << 24), PA_ERR_INVALID, "bad argument");
+        const double l = m->m->gradients_host(x, t, n, g);
+        if (loss) *loss = l;
+    });
+}
+pa_status pa_mlp_adam_step(pa_mlp* m, const float* x, const float* t, int32_t n, double lr, double* loss) {
+    MK({
+        PA_CHECK(x && t && n > 0 && n <= (1 << 24) && lr >= 0, PA_ERR_INVALID, "bad argument");
+        const double l = m->m->adam_host(x, t, n, (float)lr);
+        if (loss) *loss = l;
+    });
+}
+pa_status pa_mlp_train(pa_mlp* m, const float* samples, int64_t n, int32_t epochs, int32_t batch, double lr, uint64_t seed,
+                       double* epoch_loss) {
+    MK(PA_CHECK(samples && lr >= 0, PA_ERR_INVALID, "bad argument");
+       m->m->train_host(samples, n, epochs, batch, (float)lr, seed, epoch_loss));
+}
+pa_status pa_mlp_time(pa_mlp* m, int32_t which, int32_t rows, int32_t reps, double* ms) {
+    MK(PA_CHECK(ms && rows > 0 && rows <= (1 << 24), PA_ERR_INVALID, "bad argument"); *ms = m->m->time_ms(which, rows, reps));
 }
 
 } // extern "C"

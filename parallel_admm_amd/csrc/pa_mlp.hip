@@ -1,0 +1,583 @@
+// MLP forward, backward and Adam on MFMA (gfx950, v_mfma_f32_16x16x32_bf16).
+//
+// Every product is one "NT" GEMM, C[M][N] = sum_k A[m][k] Bt[n][k], with both
+// operands BF16 and K contiguous, FP32 accumulation, one 16x16 output tile per
+// wave and a fused epilogue:
+//   forward hidden  a = C + b, H = SiLU(a) (also H^T and SiLU'(a) when training)
+//   forward output  y = C + b, dY = (y - t) / n, loss partials
+//   backward        dA = C * SiLU'(a)  (and dA^T)
+//   weight grads    split-K partials of dA^T H (bias gradient via a ones row)
+// Activations are kept in both layouts so that each product reads K-contiguous
+// 16-byte fragments.  Partial weight gradients are summed in a fixed order, so
+// a training run is bit-reproducible.
+#include "pa_mlp.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <random>
+
+namespace pa {
+namespace {
+
+using namespace mlp;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline uint16_t to_bf16(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+
+enum { EPI_SILU = 0, EPI_OUT = 1, EPI_BWD = 2, EPI_PART = 3 };
+
+struct GemmArgs {
+    const uint16_t* A;
+    int lda;
+    const uint16_t* Bt;
+    int ldb;
+    int M, N, K, splits;
+    int train, n_real;
+    float inv_n;
+    const float* bias;
+    uint16_t* out;
+    int ldo;
+    uint16_t* outT;
+    int ldoT;
+    float* outS;
+    const float* inS;
+    int ldS;
+    float* outF;
+    const float* T;
+    double* loss_part;
+};
+
+// One 16x16 tile per wave; wave -> (split, tile m, tile n), n fastest.
+// Fragment layout of the 16x16x32 product: lane l holds row (l % 16) of A and
+// column (l % 16) of B at k = 8 (l / 16) .. +7; the accumulator of lane l
+// holds column l % 16, rows 4 (l / 16) .. +3.
+template <int EPI> __global__ void __launch_bounds__(256) k_gemm_nt(GemmArgs g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int tn_count = g.N >> 4, tiles = (g.M >> 4) * tn_count;
+    if (wave >= tiles * g.splits) return;
+    const int split = wave / tiles;
+    const int t = wave - split * tiles;
+    const int tm = t / tn_count, tn = t - tm * tn_count;
+    const int per = (g.K >> 5) / g.splits;
+    const int k0 = split * per * 32, k1 = k0 + per * 32;
+    const int r16 = lane & 15, kq = (lane >> 4) * 8;
+    const uint16_t* pa = g.A + (size_t)(tm * 16 + r16) * g.lda + kq;
+    const uint16_t* pb = g.Bt + (size_t)(tn * 16 + r16) * g.ldb + kq;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k = k0; k < k1; k += 32) {
+        const bf16x8 a = *(const bf16x8*)(pa + k);
+        const bf16x8 b = *(const bf16x8*)(pb + k);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+    const int col = tn * 16 + r16;
+    const int row0 = tm * 16 + (lane >> 4) * 4;
+    if (EPI == EPI_PART) {
+        float* o = g.outF + ((size_t)split * g.M + row0) * g.N + col;
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[(size_t)r * g.N] = acc[r];
+    } else if (EPI == EPI_SILU) {
+        const float bc = g.bias[col];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = row0 + r;
+            const float a = acc[r] + bc;
+            const float sg = 1.f / (1.f + __expf(-a));
+            const uint16_t hb = to_bf16(a * sg);
+            g.out[(size_t)row * g.ldo + col] = hb;
+            if (g.train) {
+                g.outT[(size_t)col * g.ldoT + row] = hb;
+                g.outS[(size_t)row * g.ldS + col] = sg * (1.f + a * (1.f - sg));
+            }
+        }
+    } else if (EPI == EPI_BWD) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = row0 + r;
+            const uint16_t vb = to_bf16(acc[r] * g.inS[(size_t)row * g.ldS + col]);
+            g.out[(size_t)row * g.ldo + col] = vb;
+            g.outT[(size_t)col * g.ldoT + row] = vb;
+        }
+    } else { // EPI_OUT
+        double l = 0.0;
+        if (col < OUT) {
+            const float bc = g.bias[col];
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = row0 + r;
+                const float y = acc[r] + bc;
+                g.outF[(size_t)row * OUT + col] = y;
+                if (g.train) {
+                    const float e = row < g.n_real ? y - g.T[(size_t)row * OUT + col] : 0.f;
+                    l += (double)e * (double)e;
+                    const uint16_t eb = to_bf16(e * g.inv_n);
+                    g.out[(size_t)row * g.ldo + col] = eb;
+                    g.outT[(size_t)col * g.ldoT + row] = eb;
+                }
+            }
+        }
+        if (g.train) {
+            for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+            if (lane == 0) g.loss_part[tm] = l;
+        }
+    }
+}
+
+// Rows s < n of src (stride floats apart, via idx when given) into X
+// [rows][32] and X^T [16][cap] (row 10 = 1); padded rows are zero.
+__global__ void k_pack(const float* __restrict__ src, int stride, const int32_t* __restrict__ idx, int n, int rows, int cap, int targets,
+                       uint16_t* __restrict__ X, uint16_t* __restrict__ XT, float* __restrict__ T) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= rows) return;
+    const float* f = s < n ? src + (size_t)(idx ? idx[s] : s) * stride : nullptr;
+#pragma unroll
+    for (int k = 0; k < INP; k++) X[(size_t)s * INP + k] = (f && k < IN) ? to_bf16(f[k]) : (uint16_t)0;
+#pragma unroll
+    for (int k = 0; k < INA; k++) {
+        float v = 0.f;
+        if (f) v = k < IN ? f[k] : (k == IN ? 1.f : 0.f);
+        XT[(size_t)k * cap + s] = to_bf16(v);
+    }
+    if (targets) {
+        T[2 * (size_t)s] = f ? f[IN] : 0.f;
+        T[2 * (size_t)s + 1] = f ? f[IN + 1] : 0.f;
+    }
+}
+
+struct Copies {
+    uint16_t *W1p, *W2, *W2T, *W3, *W3T, *W4p, *W4T;
+};
+
+__device__ inline void write_copies(const Copies& c, int p, float val) {
+    const uint16_t b = to_bf16(val);
+    if (p < OB1) {
+        c.W1p[(p / IN) * INP + p % IN] = b;
+    } else if (p >= OW2 && p < OB2) {
+        const int q = p - OW2, r = q / H, k = q % H;
+        c.W2[r * H + k] = b;
+        c.W2T[k * H + r] = b;
+    } else if (p >= OW3 && p < OB3) {
+        const int q = p - OW3, r = q / H, k = q % H;
+        c.W3[r * H + k] = b;
+        c.W3T[k * H + r] = b;
+    } else if (p >= OW4 && p < OB4) {
+        const int q = p - OW4, r = q / H, k = q % H;
+        c.W4p[r * H + k] = b;
+        c.W4T[k * OUTK + r] = b;
+    }
+}
+
+__global__ void k_refresh(const float* __restrict__ w, Copies c) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < NPARAM) write_copies(c, p, w[p]);
+}
+
+// Gradient of parameter p = sum over splits of its partial (fixed order);
+// mode 0 stores it, mode 1 applies Adam (Kingma and Ba, ICLR 2015) with bias
+// corrections bc1 = 1 - beta1^t, bc2 = 1 - beta2^t and refreshes the BF16 copies.
+__global__ void k_reduce(int mode, int splits, const float* __restrict__ part, int64_t off2, int64_t off3, int64_t off4, float lr,
+                         float bc1, float bc2, float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                         float* __restrict__ grad, Copies c) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NPARAM) return;
+    const float* base;
+    int rows, cols, r, k;
+    if (p < OW2) {
+        base = part;
+        rows = H;
+        cols = INA;
+        if (p < OB1) r = p / IN, k = p % IN;
+        else r = p - OB1, k = IN;
+    } else if (p < OW3) {
+        base = part + off2;
+        rows = H;
+        cols = HA;
+        if (p < OB2) r = (p - OW2) / H, k = (p - OW2) % H;
+        else r = p - OB2, k = H;
+    } else if (p < OW4) {
+        base = part + off3;
+        rows = H;
+        cols = HA;
+        if (p < OB3) r = (p - OW3) / H, k = (p - OW3) % H;
+        else r = p - OB3, k = H;
+    } else {
+        base = part + off4;
+        rows = OUTP;
+        cols = HA;
+        if (p < OB4) r = (p - OW4) / H, k = (p - OW4) % H;
+        else r = p - OB4, k = H;
+    }
+    float gsum = 0.f;
+    for (int s = 0; s < splits; s++) gsum += base[((size_t)s * rows + r) * cols + k];
+    if (mode == 0) {
+        grad[p] = gsum;
+        return;
+    }
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float mm = b1 * m[p] + (1.f - b1) * gsum;
+    const float vv = b2 * v[p] + (1.f - b2) * gsum * gsum;
+    m[p] = mm;
+    v[p] = vv;
+    const float wn = w[p] - lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+    w[p] = wn;
+    write_copies(c, p, wn);
+}
+
+// Row 128 of every transposed hidden buffer is 1 (bias gradient), rows above are 0.
+__global__ void k_ones_rows(int cap, uint16_t* __restrict__ a, uint16_t* __restrict__ b, uint16_t* __restrict__ c) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= cap) return;
+    const uint16_t one = to_bf16(1.f);
+    a[(size_t)H * cap + s] = one;
+    b[(size_t)H * cap + s] = one;
+    c[(size_t)H * cap + s] = one;
+}
+
+template <int EPI> void launch(const GemmArgs& a, hipStream_t s) {
+    PA_CHECK(a.M % 16 == 0 && a.N % 16 == 0 && a.K % 32 == 0 && a.splits >= 1 && (a.K / 32) % a.splits == 0, PA_ERR_INVALID,
+             "mlp: product shape not tiled");
+    const int waves = (a.M / 16) * (a.N / 16) * a.splits;
+    hipLaunchKernelGGL(k_gemm_nt<EPI>, dim3((waves + 3) / 4), dim3(256), 0, s, a);
+    PA_LAUNCH_CHECK();
+}
+
+} // namespace
+
+Mlp::Mlp(hipStream_t s) {
+    if (s) {
+        stream = s;
+    } else {
+        PA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    w.alloc(NPARAM);
+    adam_m.alloc(NPARAM);
+    adam_v.alloc(NPARAM);
+    grad.alloc(NPARAM);
+    W1p.alloc(H * INP);
+    W2.alloc(H * H);
+    W2T.alloc(H * H);
+    W3.alloc(H * H);
+    W3T.alloc(H * H);
+    W4p.alloc(OUTP * H);
+    W4T.alloc(H * OUTK);
+    for (DBuf<uint16_t>* b : {&W1p, &W4p, &W4T}) b->zero(stream);
+    adam_m.zero(stream);
+    adam_v.zero(stream);
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+Mlp::~Mlp() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+void Mlp::init_params(uint64_t seed) {
+    std::mt19937_64 rng(seed);
+    std::vector<float> h(NPARAM);
+    auto fill = [&](int off, int count, int fan_in) {
+        const double a = 1.0 / std::sqrt((double)fan_in);
+        std::uniform_real_distribution<double> u(-a, a);
+        for (int i = 0; i < count; i++) h[off + i] = (float)u(rng);
+    };
+    fill(OW1, H * IN, IN);
+    fill(OB1, H, IN);
+    fill(OW2, H * H, H);
+    fill(OB2, H, H);
+    fill(OW3, H * H, H);
+    fill(OB3, H, H);
+    fill(OW4, OUT * H, H);
+    fill(OB4, OUT, H);
+    set_params(h.data());
+}
+
+void Mlp::set_params(const float* host) {
+    w.upload(host, NPARAM, stream);
+    adam_m.zero(stream);
+    adam_v.zero(stream);
+    adam_t = 0;
+    Copies c{W1p.p, W2.p, W2T.p, W3.p, W3T.p, W4p.p, W4T.p};
+    hipLaunchKernelGGL(k_refresh, dim3(grid_for(NPARAM, 256)), dim3(256), 0, stream, (const float*)w.p, c);
+    PA_LAUNCH_CHECK();
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+void Mlp::get_params(float* host) {
+    w.download(host, NPARAM, stream);
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+void Mlp::ensure(int rows) {
+    PA_CHECK(rows > 0 && rows % 32 == 0, PA_ERR_INVALID, "mlp: rows must be a positive multiple of 32");
+    if (rows <= cap) return;
+    cap = rows;
+    const size_t c = (size_t)cap;
+    X.alloc(c * INP);
+    XT.alloc(c * INA);
+    for (DBuf<uint16_t>* b : {&H1, &H2, &H3, &dA1, &dA2, &dA3}) b->alloc(c * H);
+    for (DBuf<uint16_t>* b : {&H1T, &H2T, &H3T}) b->alloc(c * HA);
+    for (DBuf<uint16_t>* b : {&dA1T, &dA2T, &dA3T}) b->alloc(c * H);
+    dY.alloc(c * OUTK);
+    dYT.alloc(c * OUTP);
+    for (DBuf<float>* b : {&S1, &S2, &S3}) b->alloc(c * H);
+    Y.alloc(c * OUT);
+    T.alloc(c * OUT);
+    loss_part.alloc(c / 16);
+    part.alloc((size_t)16 * (2 * H * HA + H * INA + OUTP * HA)); // up to 16 splits
+    for (DBuf<uint16_t>* b : {&H1T, &H2T, &H3T, &dY, &dYT}) b->zero(stream);
+    hipLaunchKernelGGL(k_ones_rows, dim3(grid_for(cap, 256)), dim3(256), 0, stream, cap, H1T.p, H2T.p, H3T.p);
+    PA_LAUNCH_CHECK();
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+void Mlp::pack(const float* src, int stride, const int32_t* idx, int n, int rows, bool targets) {
+    ensure(rows);
+    PA_CHECK(n >= 0 && n <= rows, PA_ERR_INVALID, "mlp: more samples than rows");
+    hipLaunchKernelGGL(k_pack, dim3(grid_for(rows, 256)), dim3(256), 0, stream, src, stride, idx, n, rows, cap, targets ? 1 : 0, X.p, XT.p,
+                       T.p);
+    PA_LAUNCH_CHECK();
+}
+
+void Mlp::forward(int n, int rows, bool train) {
+    PA_CHECK(rows <= cap && rows % 32 == 0 && n <= rows, PA_ERR_INVALID, "mlp: bad forward shape");
+    GemmArgs a{};
+    a.M = rows;
+    a.splits = 1;
+    a.train = train ? 1 : 0;
+    a.ldoT = cap;
+    a.ldS = H;
+    const uint16_t* in[3] = {X.p, H1.p, H2.p};
+    const uint16_t* wt[3] = {W1p.p, W2.p, W3.p};
+    uint16_t* out[3] = {H1.p, H2.p, H3.p};
+    uint16_t* outT[3] = {H1T.p, H2T.p, H3T.p};
+    float* outS[3] = {S1.p, S2.p, S3.p};
+    const int bias[3] = {OB1, OB2, OB3};
+    for (int l = 0; l < 3; l++) {
+        a.A = in[l];
+        a.lda = l == 0 ? INP : H;
+        a.Bt = wt[l];
+        a.ldb = l == 0 ? INP : H;
+        a.N = H;
+        a.K = l == 0 ? INP : H;
+        a.bias = w.p + bias[l];
+        a.out = out[l];
+        a.ldo = H;
+        a.outT = outT[l];
+        a.outS = outS[l];
+        launch<EPI_SILU>(a, stream);
+    }
+    a.A = H3.p;
+    a.lda = H;
+    a.Bt = W4p.p;
+    a.ldb = H;
+    a.N = OUTP;
+    a.K = H;
+    a.bias = w.p + OB4;
+    a.outF = Y.p;
+    a.T = T.p;
+    a.out = dY.p;
+    a.ldo = OUTK;
+    a.outT = dYT.p;
+    a.n_real = n;
+    a.inv_n = n > 0 ? 1.f / (float)n : 0.f;
+    a.loss_part = loss_part.p;
+    launch<EPI_OUT>(a, stream);
+}
+
+void Mlp::backward(int rows) {
+    GemmArgs a{};
+    a.M = rows;
+    a.splits = 1;
+    a.train = 1;
+    a.ldoT = cap;
+    a.ldS = H;
+    const uint16_t* in[3] = {dY.p, dA3.p, dA2.p};
+    const uint16_t* wt[3] = {W4T.p, W3T.p, W2T.p};
+    const float* inS[3] = {S3.p, S2.p, S1.p};
+    uint16_t* out[3] = {dA3.p, dA2.p, dA1.p};
+    uint16_t* outT[3] = {dA3T.p, dA2T.p, dA1T.p};
+    for (int l = 0; l < 3; l++) {
+        a.A = in[l];
+        a.lda = l == 0 ? OUTK : H;
+        a.Bt = wt[l];
+        a.ldb = l == 0 ? OUTK : H;
+        a.N = H;
+        a.K = l == 0 ? OUTK : H;
+        a.inS = inS[l];
+        a.out = out[l];
+        a.ldo = H;
+        a.outT = outT[l];
+        launch<EPI_BWD>(a, stream);
+    }
+    int slabs = rows / 32;
+    splits = 16;
+    while (slabs % splits) splits /= 2;
+    const int64_t off2 = (int64_t)splits * H * INA, off3 = off2 + (int64_t)splits * H * HA, off4 = off3 + (int64_t)splits * H * HA;
+    GemmArgs g{};
+    g.K = rows;
+    g.splits = splits;
+    g.lda = cap;
+    g.ldb = cap;
+    // layer 4: dW4 = dY^T H3
+    g.A = dYT.p;
+    g.Bt = H3T.p;
+    g.M = OUTP;
+    g.N = HA;
+    g.outF = part.p + off4;
+    launch<EPI_PART>(g, stream);
+    g.A = dA3T.p;
+    g.Bt = H2T.p;
+    g.M = H;
+    g.N = HA;
+    g.outF = part.p + off3;
+    launch<EPI_PART>(g, stream);
+    g.A = dA2T.p;
+    g.Bt = H1T.p;
+    g.outF = part.p + off2;
+    launch<EPI_PART>(g, stream);
+    g.A = dA1T.p;
+    g.Bt = XT.p;
+    g.N = INA;
+    g.outF = part.p;
+    launch<EPI_PART>(g, stream);
+}
+
+void Mlp::reduce(int mode, float lr) {
+    const int64_t off2 = (int64_t)splits * H * INA, off3 = off2 + (int64_t)splits * H * HA, off4 = off3 + (int64_t)splits * H * HA;
+    float bc1 = 1.f, bc2 = 1.f;
+    if (mode == 1) {
+        adam_t++;
+        bc1 = (float)(1.0 - std::pow(0.9, adam_t));
+        bc2 = (float)(1.0 - std::pow(0.999, adam_t));
+    }
+    Copies c{W1p.p, W2.p, W2T.p, W3.p, W3T.p, W4p.p, W4T.p};
+    hipLaunchKernelGGL(k_reduce, dim3(grid_for(NPARAM, 256)), dim3(256), 0, stream, mode, splits, (const float*)part.p, off2, off3, off4, lr,
+                       bc1, bc2, w.p, adam_m.p, adam_v.p, grad.p, c);
+    PA_LAUNCH_CHECK();
+}
+
+double Mlp::loss(int rows) {
+    std::vector<double> h(rows / 16);
+    loss_part.download(h.data(), h.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    double s = 0;
+    for (double v : h) s += v;
+    return s;
+}
+
+void Mlp::forward_host(const float* x, int n, float* y) {
+    PA_CHECK(n > 0, PA_ERR_INVALID, "mlp: no rows");
+    const int rows = pad_rows(n);
+    ensure(rows);
+    samples.reserve((size_t)n * IN);
+    samples.upload(x, (size_t)n * IN, stream);
+    pack(samples.p, IN, nullptr, n, rows, false);
+    forward(n, rows, false);
+    Y.download(y, (size_t)n * OUT, stream);
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+static void stage_xt(DBuf<float>& buf, const float* x, const float* t, int n, hipStream_t s) {
+    std::vector<float> h((size_t)n * (IN + OUT));
+    for (int i = 0; i < n; i++) {
+        std::copy(x + (size_t)i * IN, x + (size_t)(i + 1) * IN, h.begin() + (size_t)i * (IN + OUT));
+        h[(size_t)i * (IN + OUT) + IN] = t[2 * (size_t)i];
+        h[(size_t)i * (IN + OUT) + IN + 1] = t[2 * (size_t)i + 1];
+    }
+    buf.reserve(h.size());
+    buf.upload(h.data(), h.size(), s);
+}
+
+double Mlp::gradients_host(const float* x, const float* t, int n, float* g) {
+    PA_CHECK(n > 0, PA_ERR_INVALID, "mlp: no rows");
+    const int rows = pad_rows(n);
+    ensure(rows);
+    stage_xt(samples, x, t, n, stream);
+    pack(samples.p, IN + OUT, nullptr, n, rows, true);
+    forward(n, rows, true);
+    backward(rows);
+    reduce(0, 0.f);
+    grad.download(g, NPARAM, stream);
+    return loss(rows) / (2.0 * n);
+}
+
+double Mlp::adam_host(const float* x, const float* t, int n, float lr) {
+    PA_CHECK(n > 0, PA_ERR_INVALID, "mlp: no rows");
+    const int rows = pad_rows(n);
+    ensure(rows);
+    stage_xt(samples, x, t, n, stream);
+    pack(samples.p, IN + OUT, nullptr, n, rows, true);
+    forward(n, rows, true);
+    backward(rows);
+    const double l = loss(rows) / (2.0 * n);
+    reduce(1, lr);
+    PA_HIP(hipStreamSynchronize(stream));
+    return l;
+}
+
+void Mlp::train_host(const float* samples_h, int64_t n, int epochs, int batch, float lr, uint64_t seed, double* epoch_loss) {
+    PA_CHECK(n > 0 && n <= (int64_t)1 << 30 && epochs >= 1 && batch >= 32 && batch % 32 == 0, PA_ERR_INVALID,
+             "mlp: bad training arguments");
+    samples.reserve((size_t)n * (IN + OUT));
+    samples.upload(samples_h, (size_t)n * (IN + OUT), stream);
+    perm.reserve((size_t)n);
+    ensure(batch);
+    std::vector<int32_t> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::mt19937_64 rng(seed);
+    for (int e = 0; e < epochs; e++) {
+        std::shuffle(order.begin(), order.end(), rng);
+        perm.upload(order.data(), order.size(), stream);
+        double lsum = 0;
+        int64_t seen = 0;
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int nb = (int)std::min<int64_t>(batch, n - b0);
+            const int rows = pad_rows(nb);
+            pack(samples.p, IN + OUT, perm.p + b0, nb, rows, true);
+            forward(nb, rows, true);
+            backward(rows);
+            reduce(1, lr);
+            lsum += loss(rows) / 2.0;
+            seen += nb;
+        }
+        if (epoch_loss) epoch_loss[e] = lsum / (double)seen;
+    }
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+double Mlp::time_ms(int which, int rows, int reps) {
+    PA_CHECK(reps > 0 && reps <= 100000 && (which == 0 || which == 1), PA_ERR_INVALID, "mlp: bad timing request");
+    rows = pad_rows(rows);
+    ensure(rows);
+    // a fixed synthetic batch: zero features and targets (timing only)
+    samples.reserve((size_t)rows * (IN + OUT));
+    PA_HIP(hipMemsetAsync(samples.p, 0, sizeof(float) * (size_t)rows * (IN + OUT), stream));
+    pack(samples.p, IN + OUT, nullptr, rows, rows, true);
+    std::vector<float> keep(NPARAM);
+    get_params(keep.data());
+    hipEvent_t e0, e1;
+    PA_HIP(hipEventCreate(&e0));
+    PA_HIP(hipEventCreate(&e1));
+    auto one = [&]() {
+        forward(rows, rows, which == 1);
+        if (which == 1) {
+            backward(rows);
+            reduce(1, 0.f);
+        }
+    };
+    one();
+    PA_HIP(hipEventRecord(e0, stream));
+    for (int r = 0; r < reps; r++) one();
+    PA_HIP(hipEventRecord(e1, stream));
+    PA_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    set_params(keep.data()); // the weights as before; the Adam state starts over
+    return ms / reps;
+}
+
+} // namespace pa

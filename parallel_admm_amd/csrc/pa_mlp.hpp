@@ -1,0 +1,82 @@
+// Small MLP (10 -> 128 -> 128 -> 128 -> 2, SiLU) that maps per-contact
+// features to initial contact duals.  Weights are BF16 on the device with an
+// FP32 master copy; every product runs on MFMA (v_mfma_f32_16x16x32_bf16)
+// with FP32 accumulation.  The reference has no learned component: its
+// contact duals start at zero at every detection (src/solver/contact_solver.cu:243).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "pa_common.hpp"
+
+namespace pa {
+
+namespace mlp {
+constexpr int IN = 10;    // features
+constexpr int INP = 32;   // padded input width (MFMA K)
+constexpr int INA = 16;   // transposed input rows: 10 features, a ones row, zeros
+constexpr int H = 128;    // hidden width
+constexpr int HA = 144;   // transposed hidden rows: 128, a ones row, zeros
+constexpr int OUT = 2;    // outputs
+constexpr int OUTP = 16;  // padded output width (MFMA N)
+constexpr int OUTK = 32;  // padded output-gradient width (MFMA K)
+// Flat parameter layout: W1[128][10] b1[128] W2[128][128] b2 W3[128][128] b3 W4[2][128] b4[2].
+constexpr int OW1 = 0, OB1 = OW1 + H * IN, OW2 = OB1 + H, OB2 = OW2 + H * H, OW3 = OB2 + H, OB3 = OW3 + H * H,
+              OW4 = OB3 + H, OB4 = OW4 + OUT * H, NPARAM = OB4 + OUT;
+} // namespace mlp
+
+struct Mlp {
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    // parameters: FP32 master, Adam moments, BF16 copies in the layouts the
+    // products read (row-major [out][in] and transposed [in][out])
+    DBuf<float> w, adam_m, adam_v;
+    int adam_t = 0;
+    DBuf<uint16_t> W1p, W2, W2T, W3, W3T, W4p, W4T;
+    // activations for up to `cap` rows (rows padded to a multiple of 32)
+    int cap = 0;
+    DBuf<uint16_t> X, XT, H1, H2, H3, H1T, H2T, H3T, dY, dYT, dA1, dA2, dA3, dA1T, dA2T, dA3T;
+    DBuf<float> S1, S2, S3, Y, T;
+    DBuf<float> part;        // split-K partial weight gradients
+    DBuf<double> loss_part;  // per output tile
+    DBuf<float> grad;        // flat gradient (gradient-only mode)
+    DBuf<float> samples;     // training set on the device, 12 floats per sample
+    DBuf<int32_t> perm;      // sample order of the current epoch
+    int splits = 1;
+
+    explicit Mlp(hipStream_t s = nullptr);
+    ~Mlp();
+    Mlp(const Mlp&) = delete;
+    Mlp& operator=(const Mlp&) = delete;
+
+    static int pad_rows(int n) { return ((n + 31) / 32) * 32; }
+    // Uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights and biases from a seeded generator.
+    void init_params(uint64_t seed);
+    void set_params(const float* host);
+    void get_params(float* host);
+    void ensure(int rows);
+
+    // Packs rows of `src` (stride floats apart, features first, then 2
+    // targets when `targets`) into the input buffers; idx selects rows (or null).
+    void pack(const float* src, int stride, const int32_t* idx, int n, int rows, bool targets);
+    // Forward over `rows` padded rows (n real); with `train`, keeps what the
+    // backward pass needs and forms dY = (y - t) / n and the loss partials.
+    void forward(int n, int rows, bool train);
+    void backward(int rows);
+    // Sums the split-K partials; mode 0 writes the flat gradient, mode 1 takes an Adam step.
+    void reduce(int mode, float lr);
+    double loss(int rows);
+
+    // Host-facing entry points (inputs and outputs in host memory).
+    void forward_host(const float* x, int n, float* y);
+    double gradients_host(const float* x, const float* t, int n, float* g);
+    double adam_host(const float* x, const float* t, int n, float lr);
+    void train_host(const float* samples_h, int64_t n, int epochs, int batch, float lr, uint64_t seed, double* epoch_loss);
+    // Average ms of the forward (which 0) or forward+backward+Adam (which 1)
+    // of one batch of `rows` rows, over `reps` back-to-back repetitions.  The
+    // weights are kept; the Adam state is reset.
+    double time_ms(int which, int rows, int reps);
+};
+
+} // namespace pa

@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <type_traits>
 
 namespace pa {
@@ -516,6 +517,136 @@ __global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __
     if (i >= n) return;
     if (vptr[i] == vptr[i + 1]) return;
     st3(p, i, ld3(p, i) + gather_contacts(i, vptr, vent, G, du));
+}
+
+// ---------------------------------------------------------------------------
+// Warm start of the contact duals (no counterpart in the reference, whose
+// duals restart from zero at every detection, contact_solver.cu:243).
+// ---------------------------------------------------------------------------
+__device__ inline bool key_less(int t0, int a0, int b0, int t1, int a1, int b1) {
+    return t0 != t1 ? t0 < t1 : (a0 != a1 ? a0 < a1 : b0 < b1);
+}
+
+// The 10 features of contact c at the start of its window (definitions in
+// pa_admm.h, pa_warm_start), the tangent t of the feature frame, and the index
+// of the same contact (type, prim0, prim1) in the previous window's list,
+// which is in the same canonical order, by binary search.
+__global__ void k_ws_features(int nc, int n, int nprev, float h, float dhat, float mu, const int32_t* __restrict__ type,
+                              const int32_t* __restrict__ prim0, const int32_t* __restrict__ prim1, const int4* __restrict__ cv,
+                              const float4* __restrict__ bary, const float4* __restrict__ nrm, const float4* __restrict__ K,
+                              const float* __restrict__ x, const float* __restrict__ x0, const float* __restrict__ Uc,
+                              const float* __restrict__ p, const float* __restrict__ vel, const int32_t* __restrict__ ptype,
+                              const int32_t* __restrict__ pp0, const int32_t* __restrict__ pp1, const float4* __restrict__ prc,
+                              int normalise, const float* __restrict__ consts, float* __restrict__ feat, float* __restrict__ norm,
+                              float4* __restrict__ tan, int32_t* __restrict__ match) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const float s = h / dhat, hinv = 1.f / h;
+    const int4 v = cv[c];
+    const float4 b = bary[c];
+    const int vs[4] = {v.x, v.y, v.z, v.w};
+    const float bs[4] = {b.x, b.y, b.z, b.w};
+    const float4 k4 = K[c];
+    float3 u = f3(k4.x, k4.y, k4.z), vr = f3(0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = vs[k];
+        const float3 pk = i < n ? hinv * ((ld3(x, i) - ld3(x0, i)) + ld3(Uc, i)) : ld3(p, i);
+        u = u + bs[k] * pk;
+        vr = vr + bs[k] * ld3(vel, i);
+    }
+    const float4 n4 = nrm[c];
+    const float3 nn = f3(n4.x, n4.y, n4.z);
+    const float un = dot3(nn, u);
+    const float3 ut = u - un * nn;
+    const float utn = sqrtf(dot3(ut, ut));
+    float3 t;
+    if (utn > 1e-30f) {
+        t = (1.f / utn) * ut;
+    } else { // n x (the axis of n's smallest component), normalised
+        const float ax = fabsf(nn.x), ay = fabsf(nn.y), az = fabsf(nn.z);
+        const float3 a = (ax <= ay && ax <= az) ? f3(1.f, 0.f, 0.f) : (ay <= az ? f3(0.f, 1.f, 0.f) : f3(0.f, 0.f, 1.f));
+        const float3 cr = f3(nn.y * a.z - nn.z * a.y, nn.z * a.x - nn.x * a.z, nn.x * a.y - nn.y * a.x);
+        t = (1.f / sqrtf(dot3(cr, cr))) * cr;
+    }
+    const float vn = dot3(nn, vr);
+    const float3 vt = vr - vn * nn;
+    const int kt = type[c], ka = prim0[c], kb = prim1[c];
+    int lo = 0, hi = nprev;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (key_less(ptype[mid], pp0[mid], pp1[mid], kt, ka, kb)) lo = mid + 1;
+        else hi = mid;
+    }
+    const int j = (lo < nprev && ptype[lo] == kt && pp0[lo] == ka && pp1[lo] == kb) ? lo : -1;
+    float ln = 0.f, lt = 0.f;
+    if (j >= 0) {
+        const float4 r4 = prc[j];
+        const float3 r = f3(r4.x, r4.y, r4.z);
+        ln = s * dot3(nn, r);
+        lt = s * dot3(t, r);
+    }
+    const float f[10] = {n4.w / dhat, s * un, s * utn, s * vn, s * sqrtf(dot3(vt, vt)), mu, (float)kt, j >= 0 ? 1.f : 0.f, ln, lt};
+#pragma unroll
+    for (int k = 0; k < 10; k++) {
+        feat[(size_t)c * 10 + k] = f[k];
+        if (normalise) norm[(size_t)c * 10 + k] = (f[k] - consts[k]) * consts[10 + k];
+    }
+    tan[c] = make_float4(t.x, t.y, t.z, 0.f);
+    match[c] = j;
+}
+
+// Seeds r_c: mode 1 from the network outputs y (s lambda.n, s lambda.t after
+// out_scale; non-finite -> 0, clamped), mode 2 the previous window's dual of
+// the same contact.
+__global__ void k_ws_seed(int nc, int mode, float inv_s, float clampv, const float* __restrict__ Y, const float* __restrict__ consts,
+                          const float4* __restrict__ nrm, const float4* __restrict__ tan, const int32_t* __restrict__ match,
+                          const float4* __restrict__ prc, float4* __restrict__ rc, float4* __restrict__ seed) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    float3 l = f3(0.f, 0.f, 0.f);
+    if (mode == 1) {
+        float a = Y[2 * (size_t)c] * consts[20], b = Y[2 * (size_t)c + 1] * consts[21];
+        a = isfinite(a) ? fminf(fmaxf(a, -clampv), clampv) : 0.f;
+        b = isfinite(b) ? fminf(fmaxf(b, -clampv), clampv) : 0.f;
+        const float4 n4 = nrm[c], t4 = tan[c];
+        l = inv_s * (a * f3(n4.x, n4.y, n4.z) + b * f3(t4.x, t4.y, t4.z));
+    } else {
+        const int j = match[c];
+        if (j >= 0) {
+            const float4 r = prc[j];
+            l = f3(r.x, r.y, r.z);
+        }
+    }
+    const float4 o = make_float4(l.x, l.y, l.z, 0.f);
+    rc[c] = o;
+    seed[c] = o;
+}
+
+// Judge of a seeded window: partial sums of |r_c - seed|^2 and |r_c|^2.
+__global__ void __launch_bounds__(BLK) k_ws_judge(int nc, const float4* __restrict__ rc, const float4* __restrict__ seed,
+                                                   double* __restrict__ part) {
+    double acc[2] = {0.0, 0.0};
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const float4 r = rc[c], q = seed[c];
+        const double dx = (double)r.x - q.x, dy = (double)r.y - q.y, dz = (double)r.z - q.z;
+        acc[0] += dx * dx + dy * dy + dz * dz;
+        acc[1] += (double)r.x * r.x + (double)r.y * r.y + (double)r.z * r.z;
+    }
+    block_sum_write<2>(acc, part, 2, 0);
+}
+
+// Log rows: the window's features, then s (r_c.n), s (r_c.t).
+__global__ void k_ws_log(int nc, float s, const float* __restrict__ feat, const float4* __restrict__ rc, const float4* __restrict__ nrm,
+                         const float4* __restrict__ tan, float* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    float* o = out + (size_t)c * 12;
+#pragma unroll
+    for (int k = 0; k < 10; k++) o[k] = feat[(size_t)c * 10 + k];
+    const float4 r = rc[c], n4 = nrm[c], t4 = tan[c];
+    o[10] = s * ((r.x * n4.x + r.y * n4.y) + r.z * n4.z);
+    o[11] = s * ((r.x * t4.x + r.y * t4.y) + r.z * t4.z);
 }
 
 // b = bconst + sum_e coef_e (z - Ue)[row_e] + h^2 wc (h p + x0 - Uc), accumulated in FP64.
@@ -1234,6 +1365,7 @@ Backend::~Backend() {
     // an attached external detector goes back to its own stream
     if (det && det != own_det.get()) det->stream = det->own_stream;
     own_det.reset();
+    ws_mlp.reset(); // uses the stream
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -2146,6 +2278,145 @@ void Backend::collect_phase_times() {
     ev_next = 0;
 }
 
+void Backend::set_warm_start(const pa_warm_start& w) {
+    PA_CHECK(w.mode >= 0 && w.mode <= 2, PA_ERR_INVALID, "warm start mode must be 0, 1 or 2");
+    PA_CHECK(w.mode != 1 || w.params, PA_ERR_INVALID, "warm start mode 1 needs the network parameters");
+    PA_CHECK(w.patience >= 0 && w.stall_ratio >= 0 && w.clamp >= 0, PA_ERR_INVALID, "negative warm start setting");
+    std::vector<float> consts(22);
+    for (int k = 0; k < 10; k++) {
+        consts[k] = w.in_shift[k];
+        consts[10 + k] = w.in_scale[k];
+    }
+    consts[20] = w.out_scale[0];
+    consts[21] = w.out_scale[1];
+    for (float c : consts) PA_CHECK(std::isfinite(c), PA_ERR_INVALID, "non-finite warm start normalisation");
+    if (w.mode == 1) {
+        if (!ws_mlp) ws_mlp = std::make_unique<Mlp>(stream);
+        ws_mlp->set_params(w.params);
+    }
+    ws_consts.reserve(consts.size());
+    ws_consts.upload(consts.data(), consts.size(), stream);
+    ws = w;
+    ws.params = nullptr;
+    if (ws.patience == 0) ws.patience = 2;
+    if (ws.stall_ratio == 0) ws.stall_ratio = 1e3;
+    if (ws.clamp == 0) ws.clamp = 50;
+    ws_bad = 0;
+}
+
+// Judge of a seeded window (once per window): its seeds against its final
+// duals.  A window with sum |r_final - seed|^2 > 4 sum |r_final|^2 (the seeds
+// miss by more than twice what zero would) counts as bad, and `patience` bad
+// windows in a row abandon the attempt.
+void Backend::ws_judge() {
+    if (!ws_window || !ws_window_seeded || ws_judged || ws_nc == 0) return;
+    ws_judged = true;
+    const int c = ws_nc;
+    const int blocks = std::min(grid_for(c, BLK), 1024);
+    ws_err.reserve(2 * (size_t)blocks);
+    hipLaunchKernelGGL(k_ws_judge, dim3(blocks), dim3(BLK), 0, stream, c, (const float4*)rc.p, (const float4*)ws_seed.p, ws_err.p);
+    PA_LAUNCH_CHECK();
+    std::vector<double> h(2 * (size_t)blocks);
+    ws_err.download(h.data(), h.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    double e_seed = 0, e_zero = 0;
+    for (int k = 0; k < blocks; k++) {
+        e_seed += h[2 * (size_t)k];
+        e_zero += h[2 * (size_t)k + 1];
+    }
+    const bool bad = e_seed > 4.0 * e_zero;
+    ws_bad = bad ? ws_bad + 1 : 0;
+    ws_bad_windows += bad ? 1 : 0;
+    if (ws_bad >= ws.patience) ws_restart = true;
+}
+
+// End of a detection window (before the next detection replaces the list):
+// judge the seeds against the window's final duals, log the window, and keep
+// its keys and duals for matching in the next window.
+void Backend::ws_window_end() {
+    const bool want = ws.mode != 0 || log_cap > 0;
+    if (!want || !ws_window || ws_nc == 0 || !det || det->n_contacts != ws_nc) {
+        ws_forget();
+        return;
+    }
+    const int c = ws_nc;
+    const float sc = (float)(cfg.dt / cfg.thickness);
+    ws_judge();
+    if (log_cap > 0 && log_rows < log_cap) {
+        const int take = (int)std::min<int64_t>(c, log_cap - log_rows);
+        hipLaunchKernelGGL(k_ws_log, dim3(grid_for(take, BLK)), dim3(BLK), 0, stream, take, sc, (const float*)ws_feat.p,
+                           (const float4*)rc.p, (const float4*)det->c.nrm.p, (const float4*)ws_tan.p, log_buf.p + (size_t)log_rows * 12);
+        PA_LAUNCH_CHECK();
+        log_rows += take;
+    }
+    prev_type.reserve(c);
+    prev_p0.reserve(c);
+    prev_p1.reserve(c);
+    prev_rc.reserve(c);
+    PA_HIP(hipMemcpyAsync(prev_type.p, det->c.type.p, sizeof(int32_t) * c, hipMemcpyDeviceToDevice, stream));
+    PA_HIP(hipMemcpyAsync(prev_p0.p, det->c.prim0.p, sizeof(int32_t) * c, hipMemcpyDeviceToDevice, stream));
+    PA_HIP(hipMemcpyAsync(prev_p1.p, det->c.prim1.p, sizeof(int32_t) * c, hipMemcpyDeviceToDevice, stream));
+    PA_HIP(hipMemcpyAsync(prev_rc.p, rc.p, sizeof(float4) * c, hipMemcpyDeviceToDevice, stream));
+    nc_prev = c;
+    ws_window = false;
+}
+
+// Start of a detection window (after contact_setup zeroed r_c): features, and
+// in a warm-started attempt the seeds.
+void Backend::ws_window_begin() {
+    ws_nc = nc;
+    ws_window = false;
+    ws_window_seeded = false;
+    ws_judged = false;
+    const bool want = ws.mode != 0 || log_cap > 0;
+    if (want && nc > 0) {
+        const bool use_mlp = ws_active && ws.mode == 1;
+        ws_feat.reserve(10 * (size_t)nc);
+        ws_tan.reserve(nc);
+        ws_match.reserve(nc);
+        ws_seed.reserve(nc);
+        if (use_mlp) ws_norm.reserve(10 * (size_t)nc);
+        hipLaunchKernelGGL(k_ws_features, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, nc_prev, (float)cfg.dt,
+                           (float)cfg.thickness, (float)cfg.mu, (const int32_t*)det->c.type.p, (const int32_t*)det->c.prim0.p,
+                           (const int32_t*)det->c.prim1.p, (const int4*)det->c.v.p, (const float4*)det->c.bary.p,
+                           (const float4*)det->c.nrm.p, (const float4*)kc.p, (const float*)x.p, (const float*)x0.p, (const float*)Uc.p,
+                           (const float*)p.p, (const float*)v.p, (const int32_t*)prev_type.p, (const int32_t*)prev_p0.p,
+                           (const int32_t*)prev_p1.p, (const float4*)prev_rc.p, use_mlp ? 1 : 0, (const float*)ws_consts.p, ws_feat.p,
+                           ws_norm.p, ws_tan.p, ws_match.p);
+        PA_LAUNCH_CHECK();
+        ws_window = true;
+        if (ws_active) {
+            if (use_mlp) {
+                const int rows = Mlp::pad_rows(nc);
+                ws_mlp->pack(ws_norm.p, 10, nullptr, nc, rows, false);
+                ws_mlp->forward(nc, rows, false);
+            }
+            hipLaunchKernelGGL(k_ws_seed, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, ws.mode, (float)(cfg.thickness / cfg.dt),
+                               (float)ws.clamp, use_mlp ? (const float*)ws_mlp->Y.p : nullptr, (const float*)ws_consts.p,
+                               (const float4*)det->c.nrm.p, (const float4*)ws_tan.p, (const int32_t*)ws_match.p,
+                               (const float4*)prev_rc.p, rc.p, ws_seed.p);
+            PA_LAUNCH_CHECK();
+            ws_window_seeded = true;
+            ws_seeded_count++;
+        }
+    }
+    if (seed_log) {
+        std::vector<float> h;
+        if (ws_window_seeded) {
+            std::vector<float4> q(nc);
+            ws_seed.download(q.data(), nc, stream);
+            PA_HIP(hipStreamSynchronize(stream));
+            h.resize(3 * (size_t)nc);
+            for (int c = 0; c < nc; c++) {
+                h[3 * (size_t)c] = q[c].x;
+                h[3 * (size_t)c + 1] = q[c].y;
+                h[3 * (size_t)c + 2] = q[c].z;
+            }
+        }
+        seed_log_h.push_back(std::move(h));
+    }
+}
+
 // On failure inside the step the positions roll back to the start of the
 // step (velocities are untouched until the epilogue); the duals keep the
 // iterations already applied.
@@ -2179,25 +2450,104 @@ void Backend::step_impl(pa_step_stats* st) {
     ev_next = 0;
     ev_marks.clear();
     spmv_marks.clear();
+    // A warm-started attempt can be abandoned (ws_restart): the step then
+    // re-runs cold from this snapshot, exactly as a plain step would.
+    const bool can_warm = contact && ws.mode != 0;
+    if (can_warm) {
+        auto snap = [&](DBuf<float>& dst, const DBuf<float>& src, size_t count) {
+            dst.reserve(count);
+            if (count) PA_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(float) * count, hipMemcpyDeviceToDevice, stream));
+        };
+        snap(snap_x, x, 3 * (size_t)n_all);
+        snap(snap_v, v, 3 * (size_t)n_all);
+        snap(snap_ue, Ue, 3 * (size_t)m);
+        snap(snap_uc, Uc, 3 * (size_t)n);
+    }
+    ws_bad = 0;
+    ws_bad_windows = 0;
+    ws_restart = false;
+    ws_seeded_count = 0;
+    if (seed_log) seed_log_h.clear();
+    int it = 0;
+    for (int attempt = 0;; attempt++) {
+        ws_active = can_warm && attempt == 0;
+        hipEvent_t e;
+        mark_begin(5, &e);
+        hipLaunchKernelGGL(k_begin_step, dim3(grid_for(n_all, BLK)), dim3(BLK), 0, stream, n_all, n, (float)hd, (float)cfg.g, pinned.p,
+                           x.p, v.p, x0.p, p.p, bconst.p, mass.p);
+        PA_LAUNCH_CHECK();
+        if (nnz_dm > 0) {
+            hipLaunchKernelGGL(k_spmv_add, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, dm_ptr.p, dm_col.p, dm_val.p, x0.p, bconst.p);
+            PA_LAUNCH_CHECK();
+        }
+        if (!cfg.warmstart_ue) fill<float>(Ue.p, 0.f, 3 * (int64_t)m, stream);
+        if (!cfg.warmstart_uc) fill<float>(Uc.p, 0.f, 3 * (int64_t)n, stream);
+        mark_end(5, e);
+        it = admm_loop(S, eps, gam_, hinv);
+        if (!ws_restart) break;
+        S.ws_fallback = 1;
+        S.ws_wasted_iters += it;
+        S.detections = S.global_iters = S.converged = 0;
+        S.r_primal = S.r_dual = 0;
+        auto back = [&](DBuf<float>& dst, const DBuf<float>& src, size_t count) {
+            if (count) PA_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(float) * count, hipMemcpyDeviceToDevice, stream));
+        };
+        back(x, snap_x, 3 * (size_t)n_all);
+        back(v, snap_v, 3 * (size_t)n_all);
+        back(Ue, snap_ue, 3 * (size_t)m);
+        back(Uc, snap_uc, 3 * (size_t)n);
+        ws_forget();
+        ws_restart = false;
+        ws_seeded_count = 0;
+        if (seed_log) seed_log_h.clear();
+    }
+    ws_active = false;
     hipEvent_t e;
     mark_begin(5, &e);
-    hipLaunchKernelGGL(k_begin_step, dim3(grid_for(n_all, BLK)), dim3(BLK), 0, stream, n_all, n, (float)hd, (float)cfg.g, pinned.p, x.p,
-                       v.p, x0.p, p.p, bconst.p, mass.p);
+    hipLaunchKernelGGL(k_end_step, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)hd, x.p, x0.p, v.p);
     PA_LAUNCH_CHECK();
-    if (nnz_dm > 0) {
-        hipLaunchKernelGGL(k_spmv_add, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, dm_ptr.p, dm_col.p, dm_val.p, x0.p, bconst.p);
-        PA_LAUNCH_CHECK();
-    }
-    if (!cfg.warmstart_ue) fill<float>(Ue.p, 0.f, 3 * (int64_t)m, stream);
-    if (!cfg.warmstart_uc) fill<float>(Uc.p, 0.f, 3 * (int64_t)n, stream);
     mark_end(5, e);
+    if (sync_to_host) {
+        std::vector<float> xf(3 * (size_t)n_all), vf(3 * (size_t)n_all);
+        x.download(xf.data(), xf.size(), stream);
+        v.download(vf.data(), vf.size(), stream);
+        PA_HIP(hipStreamSynchronize(stream));
+        for (size_t i = 0; i < xf.size(); i++) {
+            x_host[i] = xf[i];
+            v_host[i] = vf[i];
+        }
+    } else {
+        PA_HIP(hipStreamSynchronize(stream));
+    }
+    collect_phase_times();
+    S.iterations = it;
+    S.n_contacts = nc;
+    S.ws_seeded = ws_seeded_count;
+    S.ws_bad_windows = ws_bad_windows;
+    S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = S;
+}
+
+// The ADMM iterations of one attempt (after the step prologue); returns the
+// iterations run.  Stops early at residual_tol, or when the warm start is
+// abandoned (ws_restart).
+int Backend::admm_loop(pa_step_stats& S, float eps, float gam_, float hinv) {
+    const bool contact = cfg.enable_contact != 0;
+    hipEvent_t e;
     nc = 0;
+    double best = std::numeric_limits<double>::infinity();
     int it = 0;
     for (; it < cfg.admm_max_iter; it++) {
         if (contact && it % cfg.dcd_interval == 0) {
             mark_begin(0, &e);
+            ws_window_end();
+            if (ws_restart) {
+                mark_end(0, e);
+                break;
+            }
             det->detect(x.p);
             contact_setup(eps, gam_, hinv);
+            ws_window_begin();
             S.detections++;
             mark_end(0, e);
         }
@@ -2225,34 +2575,24 @@ void Backend::step_impl(pa_step_stats* st) {
             residual_read(&rp, &rd);
             S.r_primal = rp;
             S.r_dual = rd;
-            if (std::max(rp, rd) <= cfg.residual_tol) {
+            const double r = std::max(rp, rd);
+            if (ws_active) {
+                if (!std::isfinite(r) || r > ws.stall_ratio * best) {
+                    ws_restart = true;
+                    it++;
+                    break;
+                }
+                best = std::min(best, r);
+            }
+            if (r <= cfg.residual_tol) {
                 S.converged = 1;
                 it++;
                 break;
             }
         }
     }
-    mark_begin(5, &e);
-    hipLaunchKernelGGL(k_end_step, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)hd, x.p, x0.p, v.p);
-    PA_LAUNCH_CHECK();
-    mark_end(5, e);
-    if (sync_to_host) {
-        std::vector<float> xf(3 * (size_t)n_all), vf(3 * (size_t)n_all);
-        x.download(xf.data(), xf.size(), stream);
-        v.download(vf.data(), vf.size(), stream);
-        PA_HIP(hipStreamSynchronize(stream));
-        for (size_t i = 0; i < xf.size(); i++) {
-            x_host[i] = xf[i];
-            v_host[i] = vf[i];
-        }
-    } else {
-        PA_HIP(hipStreamSynchronize(stream));
-    }
-    collect_phase_times();
-    S.iterations = it;
-    S.n_contacts = nc;
-    S.step_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    if (st) *st = S;
+    if (ws_active && !ws_restart) ws_judge(); // the last window of the attempt
+    return it;
 }
 
 void Backend::reset(const double* x0h) {
@@ -2264,6 +2604,7 @@ void Backend::reset(const double* x0h) {
     fill<float>(Ue.p, 0.f, 3 * (int64_t)m, stream);
     fill<float>(Uc.p, 0.f, 3 * (int64_t)n, stream);
     PA_HIP(hipStreamSynchronize(stream));
+    ws_forget();
     x_host.assign(x0h, x0h + 3 * (size_t)n_all);
     std::fill(v_host.begin(), v_host.end(), 0.0);
     if (!tracks.empty()) {

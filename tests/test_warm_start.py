@@ -1,0 +1,268 @@
+"""GPU: the warm-start network kernels (BF16 MFMA, FP32 accumulation) and the
+warm-started step against the CPU oracle.
+
+Bars:
+  * forward / gradients: against the oracle with BF16 rounding at the same
+    points (differences come from FP32 vs FP64 accumulation and from
+    activations that round to the neighbouring BF16 value): outputs within
+    2e-3 of the output scale, gradient vectors within 2e-2 relative;
+    against the plain FP64 oracle within 5e-2 (BF16 error);
+  * Adam: the device update equals the oracle's Adam applied to the device
+    gradient to 1e-6 relative;
+  * features and targets logged by the backend: against the oracle's features
+    of the same contacts and state to 1e-4 (FP32 state);
+  * a warm-started step with the seeds the device used, given to the oracle:
+    within 1e-5 relative per frame, as the plain step;
+  * a deliberately wrong network: the step is re-run cold and equals a plain
+    step bit for bit.
+"""
+import numpy as np
+import pytest
+
+from helpers import drape_on_obstacle, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_mlp():
+    from oracle import mlp as om
+
+    return om
+
+
+def test_mlp_forward_matches_oracle(pa):
+    om = _oracle_mlp()
+    net = pa.MLP(seed=11)
+    w = net.params().astype(np.float64)
+    rng = np.random.default_rng(0)
+    for n in (1, 37, 100, 1000):
+        x = rng.standard_normal((n, om.IN)).astype(np.float32)
+        y = net.forward(x).astype(np.float64)
+        y16, _ = om.forward(w, x.astype(np.float64), emulate_bf16=True)
+        y64, _ = om.forward(w, x.astype(np.float64))
+        scale = np.max(np.abs(y64)) + 1e-3
+        e16 = np.max(np.abs(y - y16)) / scale
+        e64 = np.max(np.abs(y - y64)) / scale
+        print(f"forward n={n}: max err vs bf16 oracle {e16:.2e}, vs fp64 {e64:.2e} (scale {scale:.3f})")
+        assert e16 < 2e-3 and e64 < 5e-2
+
+
+def test_mlp_gradients_match_oracle(pa):
+    om = _oracle_mlp()
+    net = pa.MLP(seed=12)
+    w = net.params().astype(np.float64)
+    rng = np.random.default_rng(1)
+    for n in (32, 300):
+        x = rng.standard_normal((n, om.IN)).astype(np.float32)
+        t = rng.standard_normal((n, om.OUT)).astype(np.float32)
+        g, loss = net.gradients(x, t)
+        l16, g16 = om.loss_and_grad(w, x.astype(np.float64), t.astype(np.float64), emulate_bf16=True)
+        l64, g64 = om.loss_and_grad(w, x.astype(np.float64), t.astype(np.float64))
+        e16, e64 = rel_err(g, g16), rel_err(g, g64)
+        blocks = [(om.OW1, om.OW2), (om.OW2, om.OW3), (om.OW3, om.OW4), (om.OW4, om.NPARAM)]
+        per = [rel_err(g[a:b], g16[a:b]) for a, b in blocks]
+        print(f"grad n={n}: loss {loss:.6f} (oracle {l64:.6f}); rel err vs bf16 oracle {e16:.2e}, vs fp64 {e64:.2e}; "
+              f"per layer {', '.join(f'{p:.1e}' for p in per)}")
+        assert abs(loss - l64) <= 1e-2 * l64
+        assert e16 < 2e-2 and max(per) < 3e-2 and e64 < 5e-2
+
+
+def test_mlp_adam_step_matches_oracle(pa):
+    om = _oracle_mlp()
+    net = pa.MLP(seed=13)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((64, om.IN)).astype(np.float32)
+    t = rng.standard_normal((64, om.OUT)).astype(np.float32)
+    w0 = net.params().astype(np.float64)
+    opt = om.Adam(w0, lr=1e-3)
+    for k in range(3):
+        g, _ = net.gradients(x, t)        # the device gradient at the current weights
+        net.adam_step(x, t, 1e-3)         # the same gradient, then Adam on the device
+        w_dev = net.params().astype(np.float64)
+        w_orc = opt.step(g.astype(np.float64))
+        err = np.max(np.abs(w_dev - w_orc)) / 1e-3
+        print(f"adam step {k + 1}: max |w_dev - w_oracle| / lr = {err:.2e}")
+        assert err < 1e-3
+        opt.w = w_dev.copy()
+
+
+def test_mlp_training_fits_and_is_deterministic(pa):
+    rng = np.random.default_rng(3)
+    n = 8192
+    x = rng.standard_normal((n, 10)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1], x[:, 2] * x[:, 3] - 0.3 * x[:, 7]], axis=1).astype(np.float32)
+    samples = np.concatenate([x, t], axis=1)
+    a, b = pa.MLP(seed=5), pa.MLP(seed=5)
+    la = a.train(samples, epochs=6, batch=512, lr=2e-3, seed=9)
+    lb = b.train(samples, epochs=6, batch=512, lr=2e-3, seed=9)
+    print("epoch losses:", ", ".join(f"{v:.4f}" for v in la))
+    np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(a.params(), b.params())
+    assert la[-1] < 0.2 * la[0]
+
+
+def _one_window_cfg(name="cloth32"):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for(name)
+    cfg.dcd_interval = cfg.admm_max_iter  # one detection window per step
+    return cfg
+
+
+def test_contact_log_matches_oracle_features(pa):
+    """Backend-logged rows (features at a window's start, final duals) against
+    the oracle's features of the same contacts from the same state."""
+    om = _oracle_mlp()
+    cfg = _one_window_cfg()
+    sc = pa.Scene.generate(4, side=24, radius=0.2, drop=0.002)
+    n = sc.info["n_dyn"]
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    det = pa.CollisionDetector.for_scene(sc, capacity=cfg.max_contacts, thickness=cfg.thickness)
+    b.attach_detector(det)  # the step's contact list stays readable here
+    b.set_contact_log(1 << 16)
+    h, dhat, g = np.float32(cfg.dt), cfg.thickness, np.float32(cfg.g)
+    gam = h * cfg.kappa / (h * cfg.kappa + cfg.beta)
+    prev = None
+    checked = 0
+    for step in range(5):
+        x, v = b.get_state()
+        _, uc = b.get_duals()
+        b.step()
+        rows = b.read_contact_log(1 << 16)
+        if step > 0:  # rows of the previous step's window
+            assert rows.shape[0] == want_f.shape[0]
+            # the tangent is the direction of u0's tangential part: compare the
+            # tangent-dependent columns where that part is well above FP32 noise
+            ok = want_f[:, 2] > 1e-2 * max(want_f[:, 2].max(), 1e-30)
+            cols = [k for k in range(10) if k != 9]
+            for k in range(12):
+                got = rows[:, k]
+                ref = want_f[:, k] if k < 10 else want_t[:, k - 10]
+                print(f"  col {k}: max |got - want| {np.abs(got - ref).max():.3e}, max |want| {np.abs(ref).max():.3e}, "
+                      f"worst row {int(np.argmax(np.abs(got - ref)))}")
+            for k in cols:
+                np.testing.assert_allclose(rows[:, k], want_f[:, k], rtol=1e-4, atol=1e-4 * np.abs(want_f[:, k]).max(), err_msg=f"col {k}")
+            np.testing.assert_allclose(rows[ok, 9], want_f[ok, 9], rtol=1e-4, atol=1e-4 * (np.abs(want_f[:, 9]).max() + 1e-9))
+            np.testing.assert_allclose(rows[:, 10], want_t[:, 0], rtol=1e-4, atol=1e-4 * (np.abs(want_t[:, 0]).max() + 1e-9))
+            np.testing.assert_allclose(rows[ok, 11], want_t[ok, 1], rtol=1e-4, atol=1e-4 * (np.abs(want_t[:, 1]).max() + 1e-9))
+            checked += rows.shape[0]
+            print(f"step {step}: {rows.shape[0]} rows, {int(rows[:, 7].sum())} matched to the previous window")
+        # this step's window: features from the step-start state (FP32 prologue as on the device)
+        xf, vf = x.astype(np.float32), v.astype(np.float32)
+        vt = vf.copy()
+        vt[:n, 1] -= g * h
+        xt = xf.copy()
+        xt[:n] = xf[:n] + h * vt[:n]
+        ucf = np.zeros_like(xf)
+        ucf[:n] = uc.astype(np.float32) if cfg.warmstart_uc else 0.0
+        ints, reals = det.contacts()  # the list of this step's single window
+        Kc = (1.0 / h) * reals[:, 7:8] * gam * reals[:, 4:7]
+        f, tan, _ = om.contact_features(float(h), dhat, cfg.mu, n, xt.astype(np.float64), xf.astype(np.float64),
+                                        ucf.astype(np.float64), vt.astype(np.float64), vf.astype(np.float64), ints, reals, Kc, prev)
+        rc = b.contact_duals()
+        s = float(h) / dhat
+        want_f = f
+        want_t = np.stack([s * np.sum(rc * reals[:, 4:7], axis=1), s * np.sum(rc * tan, axis=1)], axis=1)
+        prev = (ints[:, :3].copy(), rc.copy())
+    assert checked > 0 and int(rows[:, 7].sum()) > 0
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_warm_started_step_matches_oracle_with_same_seeds(pa, mode):
+    """The seeds the device used at each detection, handed to the oracle: one
+    step from a common state within 1e-5 relative per frame."""
+    from oracle.pyoracle import Oracle
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    sc = pa.Scene.generate(4, side=32, radius=0.2, drop=0.002)
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    net = pa.MLP(seed=21)
+    b.set_warm_start(mode, params=net.params() if mode == 1 else None, patience=1000, stall_ratio=1e30)
+    b.set_seed_log(True)
+    o = Oracle(sc.mesh(), cfg)
+    worst, seeded = 0.0, 0
+    for f in range(6):
+        x, v = b.get_state()
+        ue, uc = b.get_duals()
+        o.set_state(x, v)
+        o.set_duals(ue, uc)
+        st = b.step()
+        seeds = b.read_seed_log()
+        o.set_seed_schedule(seeds)
+        so = o.step()
+        e = rel_err(b.get_state()[0], o.state()[0])
+        worst = max(worst, e)
+        seeded += sum(s is not None and len(s) > 0 for s in seeds)
+        assert st["ws_fallback"] == 0 and st["n_contacts"] == so["n_contacts"]
+        print(f"mode {mode} frame {f}: contacts {st['n_contacts']}, seeded detections {st['ws_seeded']}, "
+              f"|seed|max {max((np.abs(s).max() for s in seeds if s is not None and len(s)), default=0):.3e}, rel err {e:.2e}")
+    assert worst < 1e-5 and seeded > 0
+
+
+def test_wrong_network_falls_back_to_plain_step(pa):
+    """Output biases that seed every contact with a large wrong dual: the judge
+    (patience 1, one detection window per step) abandons every seeded attempt
+    and the step, re-run cold, equals the plain step bit for bit."""
+    cfg = _one_window_cfg()
+    sc = pa.Scene.generate(4, side=32, radius=0.2, drop=0.002)
+    plain = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    warm = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    x0 = drape_on_obstacle(sc, cfg.thickness, frac=0.3)  # in contact from the first window
+    plain.set_state(x=x0)
+    warm.set_state(x=x0)
+    w = pa.MLP(seed=22).params()
+    w[-2:] = [-40.0, 40.0]  # output biases: s lambda.n = -40, s lambda.t = +40
+    warm.set_warm_start(1, params=w, patience=1)
+    fallbacks = 0
+    for f in range(8):
+        sp = plain.step()
+        sw = warm.step()
+        fallbacks += sw["ws_fallback"]
+        xp, vp = plain.get_state()
+        xw, vw = warm.get_state()
+        print(f"frame {f}: contacts {sp['n_contacts']}, fallback {sw['ws_fallback']}, wasted iters {sw['ws_wasted_iters']}")
+        if sp["n_contacts"] > 0:
+            assert sw["ws_fallback"] == 1
+        np.testing.assert_array_equal(xw, xp)
+        np.testing.assert_array_equal(vw, vp)
+        assert sw["iterations"] == sp["iterations"]
+    assert fallbacks > 0
+
+
+def test_trained_network_gain(pa):
+    """What the committed network (parallel_admm_amd/data/warm_start_mlp.npz,
+    trained on the GPU by `python -m parallel_admm_amd.warm_start train`) gains
+    over the method without it, on scenes it was not trained on:
+      * prediction: R^2 of the window-final duals on held-out contact states
+        (generator seed 123; training used seed 0), against a zero start (the
+        reference's) and against carrying the previous window's dual;
+      * solver: ADMM iterations and final residual, plain vs warm-started.
+    The network must predict far better than both baselines; the solver must
+    not get worse by more than 10% in iterations (measured gains are small:
+    the Jacobi contact sweep re-derives the duals from the primal within about
+    one sweep, so the seed has little left to save)."""
+    from parallel_admm_amd import warm_start as ws
+    from parallel_admm_amd.configs import config_for
+
+    model = ws.Model.load()
+    rows = ws.collect_samples(60000, seed=123, steps=60, max_scenes=6, log=lambda *a: None)
+    net = pa.MLP(0)
+    net.set_params(model.params)
+    y = net.forward((rows[:, :10] - model.in_shift) * model.in_scale) * model.out_scale
+    t = rows[:, 10:12].astype(np.float64)
+    var = ((t - t.mean(axis=0)) ** 2).sum(axis=0)
+    r2 = lambda pred: 1.0 - ((pred - t) ** 2).sum(axis=0) / var
+    r2_mlp, r2_carry, r2_zero = r2(y), r2(rows[:, 8:10]), r2(np.zeros_like(t))
+    print(f"held-out {rows.shape[0]} rows: R^2 network {r2_mlp}, carry-over {r2_carry}, zero {r2_zero}")
+    assert np.all(r2_mlp > 0.5) and np.all(r2_mlp > r2_carry + 0.3) and np.all(r2_mlp > r2_zero + 0.3)
+
+    cfg = config_for("cloth32")
+    cfg.residual_tol = 1e-4
+    out = {}
+    for mode in (0, 1):
+        sc = pa.Scene.generate(4, side=40, radius=0.15, drop=0.05)
+        stats, wall, _ = ws.run(sc, cfg, 30, mode=mode, model=model, warmup=5)
+        out[mode] = ws.summary(stats, wall)
+        print(f"mode {mode}: {out[mode]}")
+    assert out[1]["admm_iters_incl_wasted"] <= 1.1 * out[0]["admm_iters_mean"]
