@@ -380,6 +380,42 @@ pa_status pa_mlp_train(pa_mlp* m, const float* samples, int64_t n, int32_t epoch
  * `rows` rows over `reps` back-to-back runs (weights kept, Adam state reset). */
 pa_status pa_mlp_time(pa_mlp* m, int32_t which, int32_t rows, int32_t reps, double* ms);
 
+/* ---------------------------------------------------------------------------
+ * Batched scenes.  No counterpart in the reference, which runs one scene per
+ * process (its solver keeps function-static state).  Scenes merged into one
+ * block-diagonal scene step together in one backend: one detection over all
+ * of them (each scene's primitives in their own LBVH subtree, so no pair spans
+ * two scenes), one pass of every per-element kernel over all rows, and the
+ * global solve as one persistent launch with a workgroup per scene
+ * (Jacobi-preconditioned CG, per-scene convergence; the workgroup shape is
+ * autotuned on the batch's own system at the first solve, or pinned).
+ * pa_backend_create_from_scene on a merged scene builds this batched backend.
+ * The ADMM stop test (residual_tol) is over the whole batch.
+ * ------------------------------------------------------------------------- */
+/* Merges 1..2048 scenes: the dynamic vertices of all scenes first (scene
+ * order), then their static vertices; elements keep their scene order. */
+pa_status pa_scene_merge(const pa_scene* const* scenes, int32_t count, pa_scene** out);
+/* Segmented detection for a merged scene: seg[v] (n_all) is the scene of
+ * vertex v (count scenes); pairs are only sought inside a scene.  Keys become
+ * (scene << 21 | 21-bit Morton code), so each scene's primitives form one
+ * subtree, and every query starts at its own scene's subtree root. */
+pa_status pa_detector_set_segments(pa_detector* d, const int32_t* seg, int32_t count);
+/* Scene of every vertex (n_all), and per scene the first dynamic and first
+ * static vertex (count + 1 entries each); any pointer may be NULL.  *count is
+ * 1 for a scene that was not merged. */
+pa_status pa_scene_get_segments(const pa_scene* s, int32_t* seg, int32_t* dyn_off, int32_t* stat_off, int32_t* count);
+/* Per scene of a batched backend: PCG iterations and device time (ms) of its
+ * workgroup in the global solves of the last step. */
+pa_status pa_backend_batch_stats(pa_backend* b, int32_t* iters, double* solve_ms);
+/* Pins the batched solve's workgroup size (256, 512 or 1024) and lanes per CSR
+ * row (1 or 4); 0, 0 re-tunes at the next solve.  Results depend on the
+ * variant at the rounding level (reduction order), so pin it for bit-exact
+ * reruns. */
+pa_status pa_backend_set_batch_tuning(pa_backend* b, int32_t threads, int32_t lanes);
+/* The variant in use (0, 0 before tuning) and the autotuner's time of each of
+ * the 6 variants (ms; zeros when pinned). */
+pa_status pa_backend_get_batch_tuning(pa_backend* b, int32_t* threads, int32_t* lanes, double* ms6);
+
 /* Message of the last failed call on this host thread (any handle). */
 const char* pa_last_error_message(void);
 

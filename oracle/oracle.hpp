@@ -1041,8 +1041,11 @@ template <class R> inline bool boxes_far(const Box<R>& a, const Box<R>& b, R gap
     return false;
 }
 
+// With `seg` (part of every vertex, for merged scenes), pairs whose
+// primitives belong to different parts are not contacts.
 template <class R>
-std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute_force) {
+std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute_force, const std::vector<int>* seg = nullptr) {
+    auto apart = [&](int a, int b) { return seg && (*seg)[a] != (*seg)[b]; };
     const int nv = (int)T.surface_vinds.size();
     const int nf = (int)T.surface_tris.size();
     const int ne = (int)T.surface_edges.size();
@@ -1077,6 +1080,7 @@ std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute
                 const auto& fa = T.surface_tris[f];
                 if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
                 if (T.is_static[v] && T.is_static[fa[0]]) continue;
+                if (apart(v, fa[0])) continue;
                 PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
                 if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
             }
@@ -1088,6 +1092,7 @@ std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute
                 const auto& e1 = T.surface_edges[j];
                 if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
                 if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
+                if (apart(e0[0], e1[0])) continue;
                 EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
                 if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
             }
@@ -1150,6 +1155,7 @@ std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute
                 const auto& fa = T.surface_tris[f];
                 if (v == fa[0] || v == fa[1] || v == fa[2]) continue;
                 if (T.is_static[v] && T.is_static[fa[0]]) continue;
+                if (apart(v, fa[0])) continue;
                 if (boxes_far(pb, fb[f], gap)) continue;
                 PTResult<R> pr = dist_pt(p, load3(x, fa[0]), load3(x, fa[1]), load3(x, fa[2]));
                 if (pr.distance <= dhat) pt_out[qi].push_back(make_contact_pt(x, v, f, fa, dhat, pr));
@@ -1177,6 +1183,7 @@ std::vector<Contact<R>> detect(const Topology& T, const R* x, R dhat, bool brute
                 const auto& e1 = T.surface_edges[j];
                 if (e0[0] == e1[0] || e0[0] == e1[1] || e0[1] == e1[0] || e0[1] == e1[1]) continue;
                 if (T.is_static[e0[0]] && T.is_static[e1[0]]) continue;
+                if (apart(e0[0], e1[0])) continue;
                 if (boxes_far(eb[i], eb[j], gap)) continue;
                 EEResult<R> er = dist_ee(load3(x, e0[0]), load3(x, e0[1]), load3(x, e1[0]), load3(x, e1[1]));
                 if (ee_accept(er, dhat)) ee_out[i].push_back(make_contact_ee(x, i, j, e0, e1, dhat, er));
@@ -1245,6 +1252,7 @@ template <class R> struct Solver {
     // contact order) replaces the zero start of r_c at the step's k-th
     // detection when its size matches; other detections start from zero.
     std::vector<std::vector<double>> seed_schedule;
+    std::vector<int> seg; // part of every vertex of a merged scene (empty: one scene)
     // Rigid tracks of static vertex ranges (ScriptAnimator rigid scripts,
     // src/solver/animator.cpp:9-39, 83-130; see add_rigid_track).
     struct Track {
@@ -1564,7 +1572,7 @@ template <class R> struct Solver {
         int it = 0, ndet = 0;
         for (; it < cfg.admm_max_iter; it++) {
             if (cfg.enable_contact && (it % cfg.dcd_interval == 0)) {
-                contacts = detect<R>(T, xc.data(), R(cfg.thickness), brute_force);
+                contacts = detect<R>(T, xc.data(), R(cfg.thickness), brute_force, seg.empty() ? nullptr : &seg);
                 compute_scc(eps, gam, hinv);
                 if (ndet < (int)seed_schedule.size() && seed_schedule[ndet].size() == r_c.size())
                     for (size_t q = 0; q < r_c.size(); q++) r_c[q] = R(seed_schedule[ndet][q]);

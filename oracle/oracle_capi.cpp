@@ -146,6 +146,12 @@ int orc_add_rigid_track(void* hp, int first, int count, int frames, const double
     return 0;
 }
 
+// Part of every vertex of a merged scene: pairs across parts are not contacts.
+void orc_set_segments(void* hp, const int* seg) {
+    auto* h = static_cast<Handle*>(hp);
+    DISPATCH(h, s.seg.assign(seg, seg + s.n_all));
+}
+
 // Seeds of the contact duals at each detection of the next step (see
 // Solver::seed_schedule): counts[k] values for detection k, concatenated.
 void orc_set_seed_schedule(void* hp, const double* seeds, const int* counts, int ndet) {
@@ -293,7 +299,7 @@ int orc_detect(void* hp, const double* x, int brute_force) {
     DISPATCH(h, {
         std::vector<R> xr(3 * (size_t)s.n_all);
         for (size_t i = 0; i < xr.size(); i++) xr[i] = R(x[i]);
-        s.contacts = detect<R>(s.T, xr.data(), R(s.cfg.thickness), brute_force != 0);
+        s.contacts = detect<R>(s.T, xr.data(), R(s.cfg.thickness), brute_force != 0, s.seg.empty() ? nullptr : &s.seg);
         nc = (int)s.contacts.size();
     });
     return nc;

@@ -40,7 +40,7 @@ def lib():
         for name in ("orc_destroy", "orc_step", "orc_sizes", "orc_get_state", "orc_set_state", "orc_get_duals", "orc_set_duals",
                      "orc_get_mass", "orc_get_topology", "orc_get_matrix", "orc_matrix_nnz", "orc_get_row_weights",
                      "orc_get_contacts", "orc_detect", "orc_project_contacts", "orc_project_elastic", "orc_solve",
-                     "orc_residual_log", "orc_set_seed_schedule"):
+                     "orc_residual_log", "orc_set_seed_schedule", "orc_set_segments"):
             getattr(L, name).argtypes = None
         _lib = L
     return _lib
@@ -122,6 +122,11 @@ class Oracle:
         a = None if ue is None else np.ascontiguousarray(ue, np.float64).reshape(-1)
         b = None if uc is None else np.ascontiguousarray(uc, np.float64).reshape(-1)
         lib().orc_set_duals(self._h, _p(a), _p(b))
+
+    def set_segments(self, seg) -> None:
+        """Part of every vertex of a merged scene (pairs across parts are skipped)."""
+        self._seg = np.ascontiguousarray(seg, np.int32)
+        lib().orc_set_segments(self._h, _p(self._seg))
 
     def set_seed_schedule(self, seeds: list) -> None:
         """Contact-dual seeds for the next step: seeds[k] = (n_c, 3) array for

@@ -41,6 +41,8 @@ EXPORTS = [
     "pa_backend_set_warm_start", "pa_backend_set_contact_log", "pa_backend_read_contact_log", "pa_backend_set_seed_log",
     "pa_backend_read_seed_log", "pa_backend_get_contact_duals", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_get_params",
     "pa_mlp_set_params", "pa_mlp_forward", "pa_mlp_gradients", "pa_mlp_adam_step", "pa_mlp_train", "pa_mlp_time",
+    "pa_scene_merge", "pa_scene_get_segments", "pa_detector_set_segments", "pa_backend_batch_stats", "pa_backend_set_batch_tuning",
+    "pa_backend_get_batch_tuning",
     "pa_last_error_message",
 ]
 
@@ -214,6 +216,26 @@ class Scene:
         _check(lib().pa_scene_create(C.byref(d), C.byref(h)), "scene create")
         return cls(h)
 
+    @classmethod
+    def merge(cls, scenes) -> "Scene":
+        """Several scenes as one block-diagonal scene (a batch): dynamic vertices
+        of all scenes first, then their static vertices."""
+        arr = (C.c_void_p * len(scenes))(*[s._h.value if isinstance(s._h, C.c_void_p) else s._h for s in scenes])
+        h = C.c_void_p()
+        _check(lib().pa_scene_merge(arr, len(scenes), C.byref(h)), "scene merge")
+        return cls(h)
+
+    def segments(self) -> dict:
+        """seg (scene of each vertex), dyn_off and stat_off (per scene, count + 1)."""
+        cnt = C.c_int32()
+        _check(lib().pa_scene_get_segments(self._h, None, None, None, C.byref(cnt)), "scene segments")
+        k = cnt.value
+        seg = np.zeros(self.info["n_all"], np.int32)
+        d = np.zeros(k + 1, np.int32)
+        s = np.zeros(k + 1, np.int32)
+        _check(lib().pa_scene_get_segments(self._h, _ptr(seg), _ptr(d), _ptr(s), C.byref(cnt)), "scene segments")
+        return dict(count=k, seg=seg, dyn_off=d, stat_off=s)
+
     def __del__(self):
         if getattr(self, "_h", None):
             lib().pa_scene_destroy(self._h)
@@ -333,6 +355,11 @@ class CollisionDetector:
 
     def clear(self) -> None:
         _check(lib().pa_detector_clear(self._h), "clear")
+
+    def set_segments(self, seg) -> None:
+        """Merged scenes: seg[v] = scene of vertex v; no pair spans two scenes."""
+        self._seg = np.ascontiguousarray(seg, np.int32)
+        _check(lib().pa_detector_set_segments(self._h, _ptr(self._seg), int(self._seg.max()) + 1), "set_segments")
 
 
 class ADMMBackendHIP:
@@ -506,6 +533,23 @@ class ADMMBackendHIP:
         self._err(lib().pa_backend_kernel_time(self._h, 1, 0, C.byref(avg), C.byref(n)), "spmv_time")
         return avg.value
 
+
+    # --- batched scenes (a backend built from a merged scene) ---
+    def batch_stats(self, count: int) -> tuple[np.ndarray, np.ndarray]:
+        """Per scene: PCG iterations and device ms of its workgroup in the last step's solves."""
+        it = np.zeros(count, np.int32)
+        ms = np.zeros(count)
+        self._err(lib().pa_backend_batch_stats(self._h, _ptr(it), _ptr(ms)), "batch_stats")
+        return it, ms
+
+    def set_batch_tuning(self, threads: int, lanes: int) -> None:
+        self._err(lib().pa_backend_set_batch_tuning(self._h, int(threads), int(lanes)), "set_batch_tuning")
+
+    def batch_tuning(self) -> dict:
+        t, l = C.c_int32(), C.c_int32()
+        ms = np.zeros(6)
+        self._err(lib().pa_backend_get_batch_tuning(self._h, C.byref(t), C.byref(l), _ptr(ms)), "batch_tuning")
+        return dict(threads=t.value, lanes=l.value, variant_ms=ms)
 
     # --- learned warm start of the contact duals ---
     def set_warm_start(self, mode: int, params=None, in_shift=None, in_scale=None, out_scale=None, patience: int = 0,

@@ -27,6 +27,7 @@ SOURCES = [
     ("pa_lbvh.hip", True, ["-ffp-contract=off"]),
     ("pa_solver.hip", True, []),
     ("pa_mlp.hip", True, []),
+    ("pa_batch.hip", True, []),
 ]
 
 

@@ -24,6 +24,8 @@ struct HostConstraints {
     int rows() const { return 2 * n_tri + 3 * n_tet + n_bend + n_pin; }
 };
 
+int batch_variant_index(int threads, int lanes); // -1 if not a variant of the batched solve
+
 struct PhaseTimer {
     bool enabled = false;
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -129,6 +131,24 @@ struct Backend {
     DBuf<float> log_buf;
     bool seed_log = false;
     std::vector<std::vector<float>> seed_log_h;
+    // batched scenes (a merged, block-diagonal scene): the global solve runs
+    // as one persistent launch with a workgroup per scene (pa_batch.hip)
+    int nscene = 0; // 0: one scene
+    std::vector<int32_t> scene_dyn_off, scene_stat_off;
+    DBuf<int32_t> scene_off, scene_order;    // dynamic row ranges; launch order (largest first)
+    DBuf<int32_t> scene_iters;               // PCG iterations per scene (to its own tolerance), summed over the step
+    DBuf<unsigned long long> scene_ticks;    // wall-clock ticks per scene in the solve, summed over the step
+    int wall_khz = 100000;
+    int tune_variant = -1;                   // chosen (threads, lanes per row) variant; -1: tune at the next solve
+    std::vector<double> tune_ms;             // the autotuner's timing of every variant (ms)
+    DBuf<float> tune_x;
+    DBuf<double> scene_r0;                   // AMG-PCG on a batch: per-scene |r0|^2 (3 per scene)
+    DBuf<uint32_t> scene_open;               // scenes not yet at tolerance
+    DBuf<int32_t> scene_first;               // iteration at which each scene met its tolerance in this pass
+    void setup_batch(const std::vector<int32_t>& dyn_off, const std::vector<int32_t>& stat_off);
+    void set_batch_tuning(int threads, int lanes);
+    int batch_solve(int max_iter, double tol);
+    void batch_stats(int32_t* iters, double* ms);
     // residual partial sums
     DBuf<double> res_part, res_sum;
     int res_blocks = 0;

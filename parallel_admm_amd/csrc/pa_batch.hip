@@ -1,0 +1,324 @@
+// Global solve of a batch of scenes in one persistent launch.
+//
+// A batch is one block-diagonal system (the scenes merged, see scene_merge),
+// so every scene's solve is independent.  One workgroup owns one scene and
+// runs its whole solve: per pass the FP64 residual r0 = b - (hi + lo) x, then
+// Jacobi-preconditioned FP32 CG on A d = r0 to |r_k| <= tol |r0| per column,
+// x += d.  Dot products are workgroup reductions, so no scene waits for
+// another: each workgroup stops when its own scene has converged (the
+// per-scene convergence mask) and the launch ends with the slowest scene.
+// Scenes are launched largest first (longest-processing-time order).  The
+// workgroup size and the lanes per CSR row are chosen by timing every variant
+// once on the batch's own system (or pinned by the caller).
+//
+// Replaces, per scene, cu_jacobi_global (src/solver/jacobi_solver.cu:141-180),
+// which the reference runs once per process and scene.
+#include "pa_backend.hpp"
+
+#include <algorithm>
+#include <numeric>
+
+namespace pa {
+namespace {
+
+struct BatchArgs {
+    const int32_t* off;   // scene row ranges (nscene + 1)
+    const int32_t* order; // launch order: workgroup -> scene
+    const int32_t* ptr;
+    const int32_t* col;
+    const float* val;
+    const float* val_lo;
+    const float* diag;
+    const float* diag_lo;
+    const float* dinv;
+    const double* b;
+    float* x;
+    float* r;
+    float* z;
+    float* p;
+    float* q;
+    float* d;
+    int32_t* iters;             // += PCG iterations per scene
+    unsigned long long* ticks;  // += wall-clock ticks per scene
+    int passes, max_iter;
+    double tol2;
+};
+
+// Workgroup sum of K doubles; every thread gets the result (fixed order).
+template <int K, int NT> __device__ inline void wg_sum(double (&v)[K]) {
+    __shared__ double sh[K][NT / 64];
+    __shared__ double fin[K];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        double s = v[k];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) sh[k][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < K) {
+        double s = 0;
+#pragma unroll 1
+        for (int w = 0; w < NT / 64; w++) s += sh[threadIdx.x][w];
+        fin[threadIdx.x] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = fin[k];
+}
+
+template <int NT, int LPR> __global__ void __launch_bounds__(NT) k_batch_pcg(BatchArgs a) {
+    const int s = a.order[blockIdx.x];
+    const int r0 = a.off[s], r1 = a.off[s + 1];
+    const unsigned long long t0 = wall_clock64();
+    const int sub = threadIdx.x % LPR;
+    constexpr int RS = NT / LPR; // rows per sweep of the CSR loops
+    int iters = 0;
+    for (int pass = 0; pass < a.passes; pass++) {
+        // FP64 residual r = b - (hi + lo) x; d = 0
+        double acc[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = r0 + (int)threadIdx.x / LPR; i < r1; i += RS) {
+            double ax[3] = {0, 0, 0};
+            for (int e = a.ptr[i] + sub; e < a.ptr[i + 1]; e += LPR) {
+                const int c = a.col[e];
+                const double v = (double)a.val[e] + (double)a.val_lo[e];
+                ax[0] += v * a.x[3 * c];
+                ax[1] += v * a.x[3 * c + 1];
+                ax[2] += v * a.x[3 * c + 2];
+            }
+#pragma unroll
+            for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+                for (int k = 0; k < 3; k++) ax[k] += __shfl_xor(ax[k], o, LPR);
+            if (sub == 0) {
+                const double di = (double)a.diag[i] + (double)a.diag_lo[i];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const double bk = a.b[3 * i + k];
+                    const double rk = bk - (ax[k] + di * a.x[3 * i + k]);
+                    a.r[3 * i + k] = (float)rk;
+                    a.d[3 * i + k] = 0.f;
+                    acc[k] += bk * bk;
+                    acc[3 + k] += rk * rk;
+                }
+            }
+        }
+        wg_sum<6, NT>(acc);
+        double thr[3], rz[3] = {0, 0, 0};
+        bool done[3];
+        for (int k = 0; k < 3; k++) {
+            thr[k] = a.tol2 * acc[3 + k];
+            done[k] = acc[3 + k] <= 1e-24 * acc[k] || acc[3 + k] == 0.0;
+        }
+        if (done[0] && done[1] && done[2]) break;
+        // z = D^-1 r, p = z, rz = r.z
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+            const float di = a.dinv[i];
+            const float3 ri = ld3(a.r, i);
+            const float3 zi = f3(di * ri.x, di * ri.y, di * ri.z);
+            st3(a.z, i, zi);
+            st3(a.p, i, zi);
+            rz[0] += (double)ri.x * zi.x;
+            rz[1] += (double)ri.y * zi.y;
+            rz[2] += (double)ri.z * zi.z;
+        }
+        wg_sum<3, NT>(rz);
+        for (int it = 0; it < a.max_iter; it++) {
+            // q = A p (FP32), p.q
+            double pq[3] = {0, 0, 0};
+            for (int i = r0 + (int)threadIdx.x / LPR; i < r1; i += RS) {
+                float3 acc3 = f3(0.f, 0.f, 0.f);
+                for (int e = a.ptr[i] + sub; e < a.ptr[i + 1]; e += LPR) {
+                    const int c = a.col[e];
+                    acc3 = acc3 + a.val[e] * ld3(a.p, c);
+                }
+#pragma unroll
+                for (int o = LPR / 2; o > 0; o >>= 1) {
+                    acc3.x += __shfl_xor(acc3.x, o, LPR);
+                    acc3.y += __shfl_xor(acc3.y, o, LPR);
+                    acc3.z += __shfl_xor(acc3.z, o, LPR);
+                }
+                if (sub == 0) {
+                    const float3 pi = ld3(a.p, i);
+                    const float3 qi = a.diag[i] * pi + acc3;
+                    st3(a.q, i, qi);
+                    pq[0] += (double)pi.x * qi.x;
+                    pq[1] += (double)pi.y * qi.y;
+                    pq[2] += (double)pi.z * qi.z;
+                }
+            }
+            wg_sum<3, NT>(pq);
+            float al[3];
+            for (int k = 0; k < 3; k++) al[k] = (done[k] || pq[k] == 0.0) ? 0.f : (float)(rz[k] / pq[k]);
+            // d += alpha p, r -= alpha q, z = D^-1 r; |r|^2 and r.z
+            double s6[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+                const float3 pi = ld3(a.p, i), qi = ld3(a.q, i);
+                float3 di = ld3(a.d, i), ri = ld3(a.r, i);
+                di = f3(di.x + al[0] * pi.x, di.y + al[1] * pi.y, di.z + al[2] * pi.z);
+                ri = f3(ri.x - al[0] * qi.x, ri.y - al[1] * qi.y, ri.z - al[2] * qi.z);
+                const float dv = a.dinv[i];
+                const float3 zi = f3(dv * ri.x, dv * ri.y, dv * ri.z);
+                st3(a.d, i, di);
+                st3(a.r, i, ri);
+                st3(a.z, i, zi);
+                s6[0] += (double)ri.x * ri.x;
+                s6[1] += (double)ri.y * ri.y;
+                s6[2] += (double)ri.z * ri.z;
+                s6[3] += (double)ri.x * zi.x;
+                s6[4] += (double)ri.y * zi.y;
+                s6[5] += (double)ri.z * zi.z;
+            }
+            wg_sum<6, NT>(s6);
+            iters++;
+            float be[3];
+            for (int k = 0; k < 3; k++) {
+                done[k] = done[k] || s6[k] <= thr[k];
+                be[k] = rz[k] == 0.0 ? 0.f : (float)(s6[3 + k] / rz[k]);
+                rz[k] = s6[3 + k];
+            }
+            if (done[0] && done[1] && done[2]) break;
+            for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+                const float3 zi = ld3(a.z, i), pi = ld3(a.p, i);
+                st3(a.p, i, f3(zi.x + be[0] * pi.x, zi.y + be[1] * pi.y, zi.z + be[2] * pi.z));
+            }
+            __syncthreads(); // p complete before the next product
+        }
+        __syncthreads();
+        for (int i = r0 + (int)threadIdx.x; i < r1; i += NT) {
+            const float3 xi = ld3(a.x, i), di = ld3(a.d, i);
+            st3(a.x, i, xi + di);
+        }
+        __syncthreads(); // x complete before the next pass's residual
+    }
+    if (threadIdx.x == 0) {
+        a.iters[s] += iters;
+        a.ticks[s] += wall_clock64() - t0;
+    }
+}
+
+struct Variant {
+    int nt, lpr;
+};
+constexpr Variant kVariants[6] = {{256, 1}, {256, 4}, {512, 1}, {512, 4}, {1024, 1}, {1024, 4}};
+
+void launch_variant(int v, int nscene, const BatchArgs& a, hipStream_t s) {
+    switch (v) {
+    case 0: hipLaunchKernelGGL((k_batch_pcg<256, 1>), dim3(nscene), dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_batch_pcg<256, 4>), dim3(nscene), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_batch_pcg<512, 1>), dim3(nscene), dim3(512), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_batch_pcg<512, 4>), dim3(nscene), dim3(512), 0, s, a); break;
+    case 4: hipLaunchKernelGGL((k_batch_pcg<1024, 1>), dim3(nscene), dim3(1024), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_batch_pcg<1024, 4>), dim3(nscene), dim3(1024), 0, s, a); break;
+    }
+    PA_LAUNCH_CHECK();
+}
+
+} // namespace
+
+int batch_variant_index(int nt, int lpr) {
+    for (int v = 0; v < 6; v++)
+        if (kVariants[v].nt == nt && kVariants[v].lpr == lpr) return v;
+    return -1;
+}
+
+void Backend::setup_batch(const std::vector<int32_t>& dyn_off, const std::vector<int32_t>& stat_off) {
+    PA_CHECK(initialized, PA_ERR_INVALID, "init first");
+    const int S = (int)dyn_off.size() - 1;
+    PA_CHECK(S >= 1 && (int)stat_off.size() == S + 1 && dyn_off[0] == 0 && dyn_off[S] == n, PA_ERR_INVALID, "bad batch layout");
+    // the system must be block-diagonal over the scenes' row ranges
+    std::vector<int32_t> ptr(n + 1), col(nnz_a_off);
+    a_ptr.download(ptr.data(), ptr.size(), stream);
+    if (nnz_a_off) a_col.download(col.data(), col.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    std::vector<int64_t> work(S);
+    for (int s = 0; s < S; s++) {
+        PA_CHECK(dyn_off[s] <= dyn_off[s + 1], PA_ERR_INVALID, "bad batch layout");
+        for (int i = dyn_off[s]; i < dyn_off[s + 1]; i++)
+            for (int e = ptr[i]; e < ptr[i + 1]; e++)
+                PA_CHECK(col[e] >= dyn_off[s] && col[e] < dyn_off[s + 1], PA_ERR_INVALID, "system couples two scenes of the batch");
+        work[s] = (int64_t)(ptr[dyn_off[s + 1]] - ptr[dyn_off[s]]) + 3 * (int64_t)(dyn_off[s + 1] - dyn_off[s]);
+    }
+    std::vector<int32_t> order(S);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    nscene = S;
+    scene_dyn_off = dyn_off;
+    scene_stat_off = stat_off;
+    scene_off.alloc(S + 1);
+    scene_off.upload(dyn_off.data(), S + 1, stream);
+    scene_order.alloc(S);
+    scene_order.upload(order.data(), S, stream);
+    scene_iters.alloc(S);
+    scene_ticks.alloc(S);
+    scene_r0.alloc(3 * (size_t)S);
+    scene_open.alloc(1);
+    scene_first.alloc(S);
+    fill<int32_t>(scene_iters.p, 0, S, stream);
+    fill<unsigned long long>(scene_ticks.p, 0ull, S, stream);
+    int khz = 0;
+    PA_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, device));
+    wall_khz = khz > 0 ? khz : 100000;
+    PA_HIP(hipStreamSynchronize(stream));
+}
+
+void Backend::set_batch_tuning(int nt, int lpr) {
+    if (nt == 0 && lpr == 0) {
+        tune_variant = -1;
+        return;
+    }
+    const int v = batch_variant_index(nt, lpr);
+    PA_CHECK(v >= 0, PA_ERR_INVALID, "batched solve variants: threads 256/512/1024, lanes per row 1/4");
+    tune_variant = v;
+    tune_ms.clear();
+}
+
+int Backend::batch_solve(int max_iter, double tol) {
+    BatchArgs a{scene_off.p, scene_order.p, a_ptr.p, a_col.p, a_val.p, a_val_lo.p, a_diag.p, a_diag_lo.p, a_dinv.p,
+                b.p,         x.p,           cg_r.p,  cg_z.p,  cg_p.p,  cg_q.p,     cg_d.p, scene_iters.p, scene_ticks.p,
+                std::max(1, cfg.global_refine), max_iter, tol * tol};
+    if (tune_variant < 0) {
+        // time every variant once on this system from the same x, then keep the fastest
+        tune_x.reserve(3 * (size_t)n);
+        PA_HIP(hipMemcpyAsync(tune_x.p, x.p, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+        hipEvent_t e0, e1;
+        PA_HIP(hipEventCreate(&e0));
+        PA_HIP(hipEventCreate(&e1));
+        tune_ms.assign(6, 0.0);
+        for (int v = 0; v < 6; v++) {
+            PA_HIP(hipMemcpyAsync(x.p, tune_x.p, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+            launch_variant(v, nscene, a, stream); // first launch of the variant: not timed
+            PA_HIP(hipMemcpyAsync(x.p, tune_x.p, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+            PA_HIP(hipEventRecord(e0, stream));
+            launch_variant(v, nscene, a, stream);
+            PA_HIP(hipEventRecord(e1, stream));
+            PA_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tune_ms[v] = ms;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        tune_variant = (int)(std::min_element(tune_ms.begin(), tune_ms.end()) - tune_ms.begin());
+        PA_HIP(hipMemcpyAsync(x.p, tune_x.p, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, stream));
+        fill<int32_t>(scene_iters.p, 0, nscene, stream); // the timing runs do not count
+        fill<unsigned long long>(scene_ticks.p, 0ull, nscene, stream);
+    }
+    launch_variant(tune_variant, nscene, a, stream);
+    return 0;
+}
+
+void Backend::batch_stats(int32_t* iters, double* ms) {
+    PA_CHECK(nscene > 0, PA_ERR_INVALID, "not a batch");
+    std::vector<int32_t> it(nscene);
+    std::vector<unsigned long long> tk(nscene);
+    scene_iters.download(it.data(), nscene, stream);
+    scene_ticks.download(tk.data(), nscene, stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    for (int s = 0; s < nscene; s++) {
+        if (iters) iters[s] = it[s];
+        if (ms) ms[s] = (double)tk[s] / (double)wall_khz;
+    }
+}
+
+} // namespace pa

@@ -1,5 +1,6 @@
 // C ABI entry points (include/pa_admm.h).  Every function converts internal
 // exceptions into a pa_status; nothing here exits the process.
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -12,6 +13,8 @@
 
 struct pa_scene {
     pa::Scene s;
+    // merged scenes only: part of every vertex, dynamic and static vertex offsets per part
+    std::vector<int32_t> seg, dyn_off, stat_off;
 };
 struct pa_detector {
     std::unique_ptr<pa::Detector> d;
@@ -139,6 +142,53 @@ pa_status pa_scene_generate(int32_t kind, int32_t side, uint64_t seed, double ra
 }
 
 void pa_scene_destroy(pa_scene* s) { delete s; }
+
+pa_status pa_detector_set_segments(pa_detector* d, const int32_t* seg, int32_t count) {
+    return guard(d ? &d->err : nullptr, [&] {
+        PA_CHECK(d && seg, PA_ERR_INVALID, "null argument");
+        d->d->set_segments(seg, count);
+    });
+}
+
+pa_status pa_scene_merge(const pa_scene* const* scenes, int32_t count, pa_scene** out) {
+    return guard(nullptr, [&] {
+        PA_CHECK(scenes && out && count >= 1 && count <= 2048, PA_ERR_INVALID, "need 1..2048 scenes");
+        std::vector<const pa::Scene*> parts(count);
+        for (int i = 0; i < count; i++) {
+            PA_CHECK(scenes[i], PA_ERR_INVALID, "null scene");
+            PA_CHECK(scenes[i]->seg.empty(), PA_ERR_INVALID, "a merged scene cannot be merged again");
+            parts[i] = &scenes[i]->s;
+        }
+        auto sc = std::make_unique<pa_scene>();
+        try {
+            sc->s = pa::scene_merge(parts, sc->seg, sc->dyn_off, sc->stat_off);
+        } catch (const std::runtime_error& e) {
+            throw pa::Error(PA_ERR_GEOMETRY, e.what());
+        }
+        *out = sc.release();
+    });
+}
+
+pa_status pa_scene_get_segments(const pa_scene* sc, int32_t* seg, int32_t* dyn_off, int32_t* stat_off, int32_t* count) {
+    return guard(nullptr, [&] {
+        PA_CHECK(sc && count, PA_ERR_INVALID, "null argument");
+        const bool merged = !sc->seg.empty();
+        *count = merged ? (int32_t)sc->dyn_off.size() - 1 : 1;
+        const int n_all = sc->s.mesh.n_all, n_dyn = sc->s.mesh.n_dyn;
+        if (seg) {
+            if (merged) std::copy(sc->seg.begin(), sc->seg.end(), seg);
+            else std::fill(seg, seg + n_all, 0);
+        }
+        if (dyn_off) {
+            if (merged) std::copy(sc->dyn_off.begin(), sc->dyn_off.end(), dyn_off);
+            else dyn_off[0] = 0, dyn_off[1] = n_dyn;
+        }
+        if (stat_off) {
+            if (merged) std::copy(sc->stat_off.begin(), sc->stat_off.end(), stat_off);
+            else stat_off[0] = n_dyn, stat_off[1] = n_all;
+        }
+    });
+}
 
 pa_status pa_scene_get_info(const pa_scene* sc, pa_scene_info* o) {
     return guard(nullptr, [&] {
@@ -432,6 +482,7 @@ pa_status pa_backend_create_from_scene(const pa_scene* sc, const pa_admm_config*
         auto bb = std::make_unique<pa_backend>();
         bb->b = std::make_unique<pa::Backend>(o);
         pa::Backend& B = *bb->b;
+        const bool batch = !sc->seg.empty();
         B.apply_config(*cfg);
         pa_constraint_soa soa;
         PA_CHECK(pa_scene_get_constraints(sc, &soa) == PA_OK, PA_ERR_INVALID, "scene constraints");
@@ -445,8 +496,10 @@ pa_status pa_backend_create_from_scene(const pa_scene* sc, const pa_admm_config*
             B.own_det = std::make_unique<pa::Detector>(s.mesh.n_all, s.mesh.n_dyn, (int)T.surf_verts.size(), T.surf_verts.data(),
                                                        (int)(T.surf_tris.size() / 3), T.surf_tris.data(), (int)(T.surf_edges.size() / 2),
                                                        T.surf_edges.data(), cfg->max_contacts, cfg->thickness, B.stream);
+            if (batch) B.own_det->set_segments(sc->seg.data(), (int)sc->dyn_off.size() - 1);
             B.attach_detector(B.own_det.get());
         }
+        if (batch) B.setup_batch(sc->dyn_off, sc->stat_off);
         *out = bb.release();
     });
 }
@@ -608,6 +661,24 @@ pa_status pa_backend_get_contact_duals(pa_backend* b, double* rco) {
             rco[3 * c + 1] = r[c].y;
             rco[3 * c + 2] = r[c].z;
         }
+    });
+}
+
+pa_status pa_backend_batch_stats(pa_backend* b, int32_t* iters, double* solve_ms) { BK(b->b->batch_stats(iters, solve_ms)); }
+
+pa_status pa_backend_set_batch_tuning(pa_backend* b, int32_t threads, int32_t lanes) {
+    BK(PA_CHECK(b->b->nscene > 0, PA_ERR_INVALID, "not a batch"); b->b->set_batch_tuning(threads, lanes));
+}
+
+pa_status pa_backend_get_batch_tuning(pa_backend* b, int32_t* threads, int32_t* lanes, double* ms6) {
+    BK({
+        pa::Backend& B = *b->b;
+        PA_CHECK(B.nscene > 0, PA_ERR_INVALID, "not a batch");
+        static const int nt[6] = {256, 256, 512, 512, 1024, 1024}, lp[6] = {1, 4, 1, 4, 1, 4};
+        if (threads) *threads = B.tune_variant >= 0 ? nt[B.tune_variant] : 0;
+        if (lanes) *lanes = B.tune_variant >= 0 ? lp[B.tune_variant] : 0;
+        if (ms6)
+            for (int v = 0; v < 6; v++) ms6[v] = v < (int)B.tune_ms.size() ? B.tune_ms[v] : 0.0;
     });
 }
 

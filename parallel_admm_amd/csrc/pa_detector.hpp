@@ -21,6 +21,7 @@ struct Tree {
     DBuf<float4> chunk4k;    // 2 per 4096-leaf chunk
     DBuf<float4> scene;      // 2: scene box
     DBuf<float4> partial;    // 2 per reduction block
+    DBuf<int32_t> seg_first, seg_last, seg_root; // per segment: sorted leaf range and subtree root (segmented mode)
 };
 
 // Contact list in canonical order (PT by (vertex, face), then EE by (edge i, edge j)).
@@ -48,6 +49,12 @@ struct Detector {
     DBuf<uint32_t> qfill;           // nv + ne: slots taken per segment by the scatter
     DBuf<uint32_t> total;           // [0] contacts found (segment counts), [1] stack overflow flag, [2] staged contacts
     PrimScratch scratch;
+    // Segmented mode (batched scenes): vertex v belongs to segment vseg[v];
+    // pairs are only sought inside a segment.  Keys are (segment << 21 |
+    // 21-bit Morton code), so each segment's leaves form one subtree of the
+    // Karras tree, and every query starts at its own segment's subtree root.
+    int nseg = 0;
+    DBuf<int32_t> vseg;
     int n_contacts = 0, peak = 0;
     int64_t found_total = 0; // contacts found by the last detection (may exceed cap)
     bool stack_overflow = false;
@@ -61,6 +68,8 @@ struct Detector {
     // the host reads the count).  Sets found_total and throws PA_ERR_OVERFLOW
     // (list unspecified) if more than `cap` contacts were found.
     int detect(const float* x);
+    // Enables segmented mode: vseg (n_all host ints) in [0, nseg), nseg <= 2048.
+    void set_segments(const int32_t* vseg_host, int nseg);
     void set_stream(hipStream_t s) { stream = s; }
 
   private:

@@ -24,6 +24,7 @@
 #include "pa_geom.hpp"
 
 #include <algorithm>
+#include <vector>
 
 #pragma clang fp contract(off)
 
@@ -122,6 +123,50 @@ __global__ void k_morton(const float4* __restrict__ box, int n, const float4* __
     uint32_t qz = quant10(ez > 0.f ? cz / ez : 0.f);
     keys[i] = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz);
     ids[i] = (uint32_t)i;
+}
+
+// Segmented keys: segment id in the top 11 bits, then a 21-bit Morton code
+// (7 bits per axis) of the box centre in the box of all segments.
+__global__ void k_morton_seg(const float4* __restrict__ box, const int32_t* __restrict__ prims, int k, int n,
+                             const int32_t* __restrict__ vseg, const float4* __restrict__ scene, uint32_t* __restrict__ keys,
+                             uint32_t* __restrict__ ids) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 slo = scene[0], shi = scene[1];
+    float4 lo = box[2 * i], hi = box[2 * i + 1];
+    float ex = shi.x - slo.x, ey = shi.y - slo.y, ez = shi.z - slo.z;
+    float cx = 0.5f * (lo.x + hi.x) - slo.x, cy = 0.5f * (lo.y + hi.y) - slo.y, cz = 0.5f * (lo.z + hi.z) - slo.z;
+    auto q7 = [](float u) { return (uint32_t)fminf(fmaxf(u * 128.0f, 0.0f), 127.0f); };
+    const uint32_t qx = q7(ex > 0.f ? cx / ex : 0.f), qy = q7(ey > 0.f ? cy / ey : 0.f), qz = q7(ez > 0.f ? cz / ez : 0.f);
+    const uint32_t m = (spread10(qx) << 2) | (spread10(qy) << 1) | spread10(qz); // 21 bits
+    keys[i] = ((uint32_t)vseg[prims[k * i]] << 21) | m;
+    ids[i] = (uint32_t)i;
+}
+
+// First and last sorted leaf of every segment (segments with no primitive keep -1).
+__global__ void k_seg_bounds(const uint32_t* __restrict__ keys, int n, int32_t* __restrict__ first, int32_t* __restrict__ last) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t s = keys[k] >> 21;
+    if (k == 0 || (keys[k - 1] >> 21) != s) first[s] = k;
+    if (k == n - 1 || (keys[k + 1] >> 21) != s) last[s] = k;
+}
+
+// The subtree root of each segment: the internal node whose leaf range is the
+// segment's range (it exists: the segment id is a key prefix), or the leaf
+// itself for a one-primitive segment.
+__global__ void k_seg_roots(int n, const int2* __restrict__ range, const uint32_t* __restrict__ keys, const int32_t* __restrict__ first,
+                            const int32_t* __restrict__ last, int32_t* __restrict__ root) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n - 1) {
+        const int2 r = range[i];
+        const uint32_t s = keys[r.x] >> 21;
+        if (first[s] == r.x && last[s] == r.y) root[s] = i;
+    }
+    if (i < n) {
+        const uint32_t s = keys[i] >> 21;
+        if (first[s] == i && last[s] == i) root[s] = (n - 1) + i;
+    }
 }
 
 __device__ inline int kdelta(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ ids, int n, int i, int j) {
@@ -310,7 +355,8 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, 
                                                    const int32_t* __restrict__ tris, int n_dyn, int nf, const float4* __restrict__ rec,
                                                    const uint32_t* __restrict__ ids, double dhat, float gap, uint32_t* __restrict__ qcount,
                                                    QueryOut st, int32_t* __restrict__ st_seg, uint32_t* __restrict__ counter,
-                                                   uint32_t* __restrict__ flags) {
+                                                   uint32_t* __restrict__ flags, const int32_t* __restrict__ vseg,
+                                                   const int32_t* __restrict__ roots) {
     __shared__ int32_t stack[STACK][QBLK];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nv) return;
@@ -338,10 +384,14 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, 
         }
         count++;
     };
-    if (nf == 1) {
-        leaf(0);
+    // start node: the root, or in segmented mode the vertex's segment subtree
+    const int start = roots ? roots[vseg[v]] : 0;
+    if (start < 0) {
+        // no triangle in this segment
+    } else if (start >= leaf0) {
+        leaf(start);
     } else {
-        int sp = 0, node = 0;
+        int sp = 0, node = start;
         while (true) {
             const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
             const int l = __float_as_int(a.w), r = __float_as_int(b.w);
@@ -378,7 +428,8 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, 
                                                    const float4* __restrict__ rec, const uint32_t* __restrict__ ids, double dhat,
                                                    float gap, uint32_t* __restrict__ qcount, int seg0, QueryOut st,
                                                    int32_t* __restrict__ st_seg, uint32_t* __restrict__ counter,
-                                                   uint32_t* __restrict__ flags) {
+                                                   uint32_t* __restrict__ flags, const int32_t* __restrict__ vseg,
+                                                   const int32_t* __restrict__ roots) {
     __shared__ int32_t stack[STACK][QBLK];
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ne || ne < 2) return;
@@ -406,7 +457,11 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, 
             st_seg[idx] = seg0 + i;
         }
     };
-    int sp = 0, node = 0;
+    // start node: the root, or in segmented mode the edge's segment subtree
+    // (a segment with one edge has no pair)
+    const int start = roots ? roots[vseg[a0]] : 0;
+    if (start < 0 || start >= leaf0) return;
+    int sp = 0, node = start;
     while (true) {
         const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
         const int l = __float_as_int(a.w), r = __float_as_int(b.w);
@@ -540,9 +595,16 @@ void Detector::build_tree(Tree& T, const float* x, bool faces) {
     PA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_box_union, dim3(1), dim3(BLK), 0, stream, T.partial.p, nb, T.scene.p);
     PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_morton, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, n, T.scene.p, T.keys.p, T.ids.p);
-    PA_LAUNCH_CHECK();
-    radix_sort_pairs(T.keys.p, T.ids.p, n, 30, scratch, stream);
+    if (nseg > 0) {
+        hipLaunchKernelGGL(k_morton_seg, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, faces ? tris.p : edges.p,
+                           faces ? 3 : 2, n, (const int32_t*)vseg.p, T.scene.p, T.keys.p, T.ids.p);
+        PA_LAUNCH_CHECK();
+        radix_sort_pairs(T.keys.p, T.ids.p, n, 32, scratch, stream);
+    } else {
+        hipLaunchKernelGGL(k_morton, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, n, T.scene.p, T.keys.p, T.ids.p);
+        PA_LAUNCH_CHECK();
+        radix_sort_pairs(T.keys.p, T.ids.p, n, 30, scratch, stream);
+    }
     hipLaunchKernelGGL(k_leaf_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, T.ids.p, n, T.node_box.p);
     PA_LAUNCH_CHECK();
     if (n > 1) {
@@ -561,6 +623,38 @@ void Detector::build_tree(Tree& T, const float* x, bool faces) {
                            T.rec.p);
         PA_LAUNCH_CHECK();
     }
+    if (nseg > 0) {
+        fill<int32_t>(T.seg_first.p, -1, nseg, stream);
+        fill<int32_t>(T.seg_last.p, -1, nseg, stream);
+        fill<int32_t>(T.seg_root.p, -1, nseg, stream);
+        hipLaunchKernelGGL(k_seg_bounds, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, (const uint32_t*)T.keys.p, n, T.seg_first.p,
+                           T.seg_last.p);
+        PA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_seg_roots, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const int2*)T.range.p, (const uint32_t*)T.keys.p,
+                           (const int32_t*)T.seg_first.p, (const int32_t*)T.seg_last.p, T.seg_root.p);
+        PA_LAUNCH_CHECK();
+    }
+}
+
+void Detector::set_segments(const int32_t* vs, int count) {
+    PA_CHECK(count >= 1 && count <= 2048, PA_ERR_INVALID, "segment count must be in [1, 2048]");
+    for (int i = 0; i < n_all; i++) PA_CHECK(vs[i] >= 0 && vs[i] < count, PA_ERR_INVALID, "vertex segment out of range");
+    // a primitive's vertices must share one segment (its key takes the first vertex's)
+    std::vector<int32_t> t(3 * (size_t)nf), e(2 * (size_t)ne);
+    tris.download(t.data(), t.size(), stream);
+    edges.download(e.data(), e.size(), stream);
+    PA_HIP(hipStreamSynchronize(stream));
+    for (int f = 0; f < nf; f++)
+        PA_CHECK(vs[t[3 * f]] == vs[t[3 * f + 1]] && vs[t[3 * f]] == vs[t[3 * f + 2]], PA_ERR_INVALID, "triangle spans two segments");
+    for (int k = 0; k < ne; k++) PA_CHECK(vs[e[2 * k]] == vs[e[2 * k + 1]], PA_ERR_INVALID, "edge spans two segments");
+    vseg.alloc(n_all);
+    vseg.upload(vs, n_all, stream);
+    nseg = count;
+    for (Tree* T : {&ft, &et}) {
+        T->seg_first.alloc(count);
+        T->seg_last.alloc(count);
+        T->seg_root.alloc(count);
+    }
 }
 
 int Detector::detect(const float* x) {
@@ -575,12 +669,14 @@ int Detector::detect(const float* x) {
     if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
     if (nv > 0) {
         hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, x, surf_verts.p, nv, tris.p, n_dyn, nf, ft.rec.p,
-                           ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p);
+                           ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
+                           nseg > 0 ? (const int32_t*)ft.seg_root.p : nullptr);
         PA_LAUNCH_CHECK();
     }
     if (ne > 0) {
         hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, x, edges.p, ne, n_dyn, et.rec.p, et.ids.p, dhat,
-                           gap, qcount.p + nv, nv, st, st_seg.p, total.p + 2, total.p);
+                           gap, qcount.p + nv, nv, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
+                           nseg > 0 ? (const int32_t*)et.seg_root.p : nullptr);
         PA_LAUNCH_CHECK();
     }
     scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);

@@ -9,6 +9,7 @@
 // is its sorted unique neighbour list (igl::adjacency_list, sorted = false).
 // Constraint rows: tri (2), tet (3), bend (1), pin (1)
 // (src/constraint/mesh_to_constraint.h:29-153, app_initializer.cpp:145-172).
+#include <climits>
 #include "pa_scene.hpp"
 
 #include <algorithm>
@@ -465,6 +466,54 @@ Scene scene_generate(int kind, int side, uint64_t seed, double radius, double dr
     } else {
         throw std::runtime_error("unknown scene kind");
     }
+    scene_finalize(s);
+    return s;
+}
+
+Scene scene_merge(const std::vector<const Scene*>& parts, std::vector<int32_t>& seg, std::vector<int32_t>& dyn_off,
+                  std::vector<int32_t>& stat_off) {
+    if (parts.empty()) throw std::runtime_error("no scenes to merge");
+    const size_t P = parts.size();
+    dyn_off.assign(P + 1, 0);
+    stat_off.assign(P + 1, 0);
+    int64_t nd = 0, ns = 0;
+    for (size_t p = 0; p < P; p++) {
+        nd += parts[p]->mesh.n_dyn;
+        ns += parts[p]->mesh.n_all - parts[p]->mesh.n_dyn;
+    }
+    if (nd + ns > INT32_MAX / 4) throw std::runtime_error("merged scene too large");
+    Scene s;
+    SceneMesh& M = s.mesh;
+    M.n_dyn = (int)nd;
+    M.n_all = (int)(nd + ns);
+    M.x.resize(3 * (size_t)M.n_all);
+    M.density.resize(M.n_all);
+    M.bend_k.resize(M.n_all);
+    seg.resize(M.n_all);
+    int d0 = 0, s0 = (int)nd;
+    for (size_t p = 0; p < P; p++) {
+        const SceneMesh& Q = parts[p]->mesh;
+        dyn_off[p] = d0;
+        stat_off[p] = s0;
+        auto map = [&](int v) { return v < Q.n_dyn ? d0 + v : s0 + (v - Q.n_dyn); };
+        for (int v = 0; v < Q.n_all; v++) {
+            const int g = map(v);
+            for (int k = 0; k < 3; k++) M.x[3 * (size_t)g + k] = Q.x[3 * (size_t)v + k];
+            M.density[g] = Q.density[v];
+            M.bend_k[g] = Q.bend_k[v];
+            seg[g] = (int32_t)p;
+        }
+        for (int32_t v : Q.tris) M.tris.push_back(map(v));
+        M.tri_k.insert(M.tri_k.end(), Q.tri_k.begin(), Q.tri_k.end());
+        for (int32_t v : Q.tets) M.tets.push_back(map(v));
+        M.tet_k.insert(M.tet_k.end(), Q.tet_k.begin(), Q.tet_k.end());
+        for (int32_t v : Q.tet_boundary) M.tet_boundary.push_back(map(v));
+        for (int32_t v : Q.pins) M.pins.push_back(map(v));
+        d0 += Q.n_dyn;
+        s0 += Q.n_all - Q.n_dyn;
+    }
+    dyn_off[P] = d0;
+    stat_off[P] = s0;
     scene_finalize(s);
     return s;
 }

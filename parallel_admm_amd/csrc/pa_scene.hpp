@@ -51,4 +51,12 @@ struct Scene {
 void scene_finalize(Scene& s);
 Scene scene_generate(int kind, int side, uint64_t seed, double radius, double drop);
 
+// Several scenes as one block-diagonal scene: the dynamic vertices of all
+// parts first (part order), then their static vertices; elements keep their
+// part order.  The result is finalized like any scene.  seg[v] = part of
+// merged vertex v; part p owns dynamic vertices [dyn_off[p], dyn_off[p+1])
+// and static vertices [stat_off[p], stat_off[p+1]).
+Scene scene_merge(const std::vector<const Scene*>& parts, std::vector<int32_t>& seg, std::vector<int32_t>& dyn_off,
+                  std::vector<int32_t>& stat_off);
+
 } // namespace pa

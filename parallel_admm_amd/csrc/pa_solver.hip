@@ -2173,6 +2173,60 @@ hipGraphExec_t Backend::pcg_pair_graph(bool first_pair) {
 // chunk sized by the previous solve's count); on the fused path a chunk is a
 // sequence of graph-launched iteration pairs.  Returns the PCG iterations
 // summed over passes.
+// Batched scenes on the global AMG-PCG: per-scene |r|^2 (one workgroup per
+// scene).  k_scene_norms keeps them as the pass's initial residuals;
+// k_scene_check counts the scenes with a column above tol^2 |r0|^2 and records,
+// per scene, the iteration at which it first met its tolerance.
+__global__ void __launch_bounds__(BLK) k_scene_norms(const int32_t* __restrict__ off, const float* __restrict__ r,
+                                                      double* __restrict__ out) {
+    const int s = blockIdx.x;
+    double acc[3] = {0, 0, 0};
+    for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += BLK) {
+        const float3 ri = ld3(r, i);
+        acc[0] += (double)ri.x * ri.x;
+        acc[1] += (double)ri.y * ri.y;
+        acc[2] += (double)ri.z * ri.z;
+    }
+    block_sum_write<3>(acc, out, 3, 0);
+}
+
+__global__ void __launch_bounds__(BLK) k_scene_check(const int32_t* __restrict__ off, const float* __restrict__ r,
+                                                      const double* __restrict__ r0, double tol2, int it, int32_t* __restrict__ first_it,
+                                                      uint32_t* __restrict__ open) {
+    __shared__ double sh[3][BLK / 64];
+    const int s = blockIdx.x;
+    double acc[3] = {0, 0, 0};
+    for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += BLK) {
+        const float3 ri = ld3(r, i);
+        acc[0] += (double)ri.x * ri.x;
+        acc[1] += (double)ri.y * ri.y;
+        acc[2] += (double)ri.z * ri.z;
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < 3; k++) {
+        double v = acc[k];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) sh[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool ok = true;
+        for (int k = 0; k < 3; k++) {
+            double v = 0;
+            for (int w = 0; w < BLK / 64; w++) v += sh[k][w];
+            ok = ok && v <= tol2 * r0[3 * s + k];
+        }
+        if (!ok) atomicAdd(open, 1u);
+        else if (first_it[s] < 0) first_it[s] = it;
+    }
+}
+
+// iters[s] += the iteration at which scene s met its tolerance (all of them if it did not)
+__global__ void k_scene_accum(int nscene, const int32_t* __restrict__ first, int total, int32_t* __restrict__ iters) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < nscene) iters[s] += first[s] >= 0 ? first[s] : total;
+}
+
 int Backend::global_solve(int max_iter, double tol) {
     const int nb = cg_blocks;
     const int passes = std::max(1, cfg.global_refine);
@@ -2182,6 +2236,17 @@ int Backend::global_solve(int max_iter, double tol) {
     const bool graphs = fused && use_graphs && !(profiling & 1);
     int total_iters = 0;
     double hs[24];
+    // Batched scenes: the stop test is per scene (each scene's own |r| against
+    // its own |r0|), so the whole-system thresholds are off (zero).
+    const bool per_scene = nscene > 0;
+    uint32_t open_scenes = 0;
+    auto scene_check = [&](const float* rcur, int it) {
+        fill<uint32_t>(scene_open.p, 0u, 1, stream);
+        hipLaunchKernelGGL(k_scene_check, dim3(nscene), dim3(BLK), 0, stream, (const int32_t*)scene_off.p, rcur, (const double*)scene_r0.p,
+                           tol * tol, it, scene_first.p, scene_open.p);
+        PA_LAUNCH_CHECK();
+        PA_HIP(hipMemcpyAsync(&open_scenes, scene_open.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    };
     for (int pass = 0; pass < passes; pass++) {
         with_lpr(a_lpr, [&](auto c) {
             constexpr int P = decltype(c)::value;
@@ -2189,8 +2254,14 @@ int Backend::global_solve(int max_iter, double tol) {
                                a_diag_lo.p, b.p, x.p, cg_r.p, cg_d.p, cg_part.p);
         });
         PA_LAUNCH_CHECK();
+        if (per_scene) {
+            hipLaunchKernelGGL(k_scene_norms, dim3(nscene), dim3(BLK), 0, stream, (const int32_t*)scene_off.p, (const float*)cg_r.p,
+                               scene_r0.p);
+            PA_LAUNCH_CHECK();
+            fill<int32_t>(scene_first.p, -1, nscene, stream);
+        }
         precondition(cg_r.p);
-        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(PBLK), 0, stream, cg_part.p, nb, tol * tol, cg_state.p);
+        hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(PBLK), 0, stream, cg_part.p, nb, per_scene ? 0.0 : tol * tol, cg_state.p);
         PA_LAUNCH_CHECK();
         if (!fused) {
             hipLaunchKernelGGL(k_copy, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, 3 * (int64_t)n, cg_z.p, cg_p.p);
@@ -2222,12 +2293,19 @@ int Backend::global_solve(int max_iter, double tol) {
                 PA_LAUNCH_CHECK();
             }
             PA_HIP(hipMemcpyAsync(hs, cg_state.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+            if (per_scene) scene_check(fused ? (const float*)(rc ? cg_r2.p : cg_r.p) : (const float*)cg_r.p, it);
             PA_HIP(hipStreamSynchronize(stream));
             const double* sl = hs + (it & 1) * 8;
             if (sl[3] != 0 && sl[4] != 0 && sl[5] != 0) break;
+            if (per_scene && open_scenes == 0) break;
         }
         if (pass == 0) last_pcg_iters = (int)hs[19];
         total_iters += (int)hs[19];
+        if (per_scene) {
+            hipLaunchKernelGGL(k_scene_accum, dim3(grid_for(nscene, BLK)), dim3(BLK), 0, stream, nscene, (const int32_t*)scene_first.p,
+                               (int)hs[19], scene_iters.p);
+            PA_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(k_axpy_d, dim3(grid_for(3 * (int64_t)n, BLK)), dim3(BLK), 0, stream, n, cg_d.p, x.p);
         PA_LAUNCH_CHECK();
     }
@@ -2463,6 +2541,10 @@ void Backend::step_impl(pa_step_stats* st) {
         snap(snap_ue, Ue, 3 * (size_t)m);
         snap(snap_uc, Uc, 3 * (size_t)n);
     }
+    if (nscene) { // per-scene solve statistics of this step
+        fill<int32_t>(scene_iters.p, 0, nscene, stream);
+        fill<unsigned long long>(scene_ticks.p, 0ull, nscene, stream);
+    }
     ws_bad = 0;
     ws_bad_windows = 0;
     ws_restart = false;
@@ -2520,6 +2602,11 @@ void Backend::step_impl(pa_step_stats* st) {
         PA_HIP(hipStreamSynchronize(stream));
     }
     collect_phase_times();
+    if (nscene) { // the critical path: the most PCG iterations of any scene
+        std::vector<int32_t> its(nscene);
+        batch_stats(its.data(), nullptr);
+        S.global_iters = *std::max_element(its.begin(), its.end());
+    }
     S.iterations = it;
     S.n_contacts = nc;
     S.ws_seeded = ws_seeded_count;
@@ -2565,7 +2652,8 @@ int Backend::admm_loop(pa_step_stats& S, float eps, float gam_, float hinv) {
         rhs(check, it == 0);
         mark_end(3, e);
         mark_begin(4, &e);
-        S.global_iters += global_solve(cfg.global_max_iter, cfg.global_tol);
+        S.global_iters += (nscene && !use_amg) ? batch_solve(cfg.global_max_iter, cfg.global_tol)
+                                               : global_solve(cfg.global_max_iter, cfg.global_tol);
         mark_end(4, e);
         mark_begin(1, &e);
         dual_update(check);

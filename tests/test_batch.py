@@ -1,0 +1,162 @@
+"""GPU: batched scenes (a merged, block-diagonal scene stepped by one backend).
+
+Bars: the segmented LBVH's contact list bit-identical to the oracle's (with
+its per-scene filter); one batched step from a common state within 1e-5
+relative of the FP64 oracle per frame, and of each scene stepped alone; every
+variant of the persistent per-scene solve within 1e-6 of the others.
+"""
+import numpy as np
+import pytest
+
+from helpers import drape_on_obstacle, rel_err
+
+pytestmark = pytest.mark.gpu
+
+PARTS = [dict(side=12, radius=0.10, drop=0.05), dict(side=20, radius=0.15, drop=0.02), dict(side=9, radius=0.20, drop=0.1),
+         dict(side=16, radius=0.12, drop=0.01)]
+
+
+def _batch(pa, cfg, frac=0.3):
+    parts = [pa.Scene.generate(4, **p) for p in PARTS]
+    merged = pa.Scene.merge(parts)
+    seg = merged.segments()
+    xm = np.zeros((merged.info["n_all"], 3))
+    xs = []
+    for s, p in enumerate(parts):
+        x = drape_on_obstacle(p, cfg.thickness, frac=frac)
+        n = p.info["n_dyn"]
+        d0, s0 = seg["dyn_off"][s], seg["stat_off"][s]
+        xm[d0:d0 + n] = x[:n]
+        xm[s0:s0 + len(x) - n] = x[n:]
+        xs.append(x)
+    return parts, merged, seg, xm, xs
+
+
+def _split(seg, s, part, xm):
+    n = part.info["n_dyn"]
+    d0, s0 = seg["dyn_off"][s], seg["stat_off"][s]
+    return np.concatenate([xm[d0:d0 + n], xm[s0:s0 + part.info["n_all"] - n]])
+
+
+def test_segmented_detection_matches_oracle(pa):
+    from oracle.pyoracle import Oracle
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    parts, merged, seg, xm, _ = _batch(pa, cfg)
+    xm = xm.astype(np.float32).astype(np.float64)
+    det = pa.CollisionDetector.for_scene(merged, capacity=1 << 18, thickness=cfg.thickness)
+    det.set_segments(seg["seg"])
+    n = det.detect(xm)
+    gi, gr = det.contacts()
+    o = Oracle(merged.mesh(), cfg)
+    o.set_segments(seg["seg"])
+    o.detect(xm, brute_force=True)
+    oi, orr = o.contacts()
+    print(f"batch of {len(parts)}: {n} contacts (oracle {len(oi)}), PT {np.sum(gi[:, 0] == 0)}, EE {np.sum(gi[:, 0] == 1)}")
+    assert n > 0 and gi.shape == oi.shape
+    np.testing.assert_array_equal(gi, oi)
+    np.testing.assert_array_equal(gr[:, :8], orr[:, :8].astype(np.float32).astype(np.float64))
+    # no pair spans two scenes
+    s = seg["seg"]
+    assert np.all(s[gi[:, 3]] == s[gi[:, 4]]) and np.all(s[gi[:, 3]] == s[gi[:, 6]])
+
+
+@pytest.mark.parametrize("precond", [1, 0], ids=["amg-whole-batch", "jacobi-per-scene"])
+def test_batched_step_matches_oracle_and_solo(pa, precond):
+    """precond 1: AMG-PCG over the whole block-diagonal system, stopped when
+    every scene meets its own tolerance; precond 0: the persistent per-scene
+    Jacobi-PCG launch."""
+    from oracle.pyoracle import Oracle
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.precond = precond
+    parts, merged, seg, xm, xs = _batch(pa, cfg, frac=2.0)
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_batch_tuning(256, 1)
+    b.set_state(x=xm)
+    o = Oracle(merged.mesh(), cfg)
+    o.set_segments(seg["seg"])
+    solo = [pa.ADMMBackendHIP.from_scene(p, cfg) for p in parts]
+    worst_o = worst_s = 0.0
+    contacts = 0
+    for f in range(5):
+        x, v = b.get_state()
+        ue, uc = b.get_duals()
+        o.set_state(x, v)
+        o.set_duals(ue, uc)
+        for s, (p, sb) in enumerate(zip(parts, solo)):
+            sb.set_state(_split(seg, s, p, x), _split(seg, s, p, v))
+            n = p.info["n_dyn"]
+            d0 = seg["dyn_off"][s]
+            sb.set_duals(None, uc[d0:d0 + n])
+        st = b.step()
+        so = o.step()
+        xg, _ = b.get_state()
+        xo, _ = o.state()
+        e = rel_err(xg, xo)
+        worst_o = max(worst_o, e)
+        it, ms = b.batch_stats(len(parts))
+        es = []
+        for s, (p, sb) in enumerate(zip(parts, solo)):
+            sb.step()
+            es.append(rel_err(_split(seg, s, p, xg), sb.get_state()[0]))
+        worst_s = max(worst_s, max(es))
+        contacts += st["n_contacts"]
+        assert st["n_contacts"] == so["n_contacts"]
+        print(f"frame {f}: contacts {st['n_contacts']}, rel err vs oracle {e:.2e}, vs solo max {max(es):.2e}, "
+              f"PCG iters per scene {it.tolist()}, solve ms per scene {np.round(ms, 3).tolist()}")
+    assert worst_o < 1e-5 and worst_s < 1e-5 and contacts > 0
+
+
+def test_batched_solve_variants_agree_and_autotune(pa):
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.precond = 0  # the persistent per-scene launch
+    parts, merged, seg, xm, _ = _batch(pa, cfg, frac=2.0)
+    finals = []
+    for threads, lanes in ((256, 1), (256, 4), (512, 1), (512, 4), (1024, 1), (1024, 4)):
+        b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+        b.set_batch_tuning(threads, lanes)
+        b.set_state(x=xm)
+        for _ in range(2):
+            b.step()
+        finals.append(b.get_state()[0])
+    for k, xf in enumerate(finals[1:], 1):
+        e = rel_err(xf, finals[0])
+        print(f"variant {k}: rel diff to variant 0 {e:.2e}")
+        assert e < 1e-6
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_state(x=xm)
+    for _ in range(2):
+        b.step()
+    t = b.batch_tuning()
+    print(f"autotuner: threads {t['threads']}, lanes {t['lanes']}, variant ms {np.round(t['variant_ms'], 3).tolist()}")
+    assert t["threads"] in (256, 512, 1024) and t["lanes"] in (1, 4) and np.all(t["variant_ms"] > 0)
+    assert rel_err(b.get_state()[0], finals[0]) < 1e-5
+
+
+@pytest.mark.parametrize("precond", [1, 0], ids=["amg-whole-batch", "jacobi-per-scene"])
+def test_batch_of_uneven_scenes_runs(pa, precond):
+    """64 scenes with sides 16..96 dropped on spheres: every scene's solve
+    converges within the iteration cap."""
+    from parallel_admm_amd.configs import config_for
+    from parallel_admm_amd.warm_start import batch_member_scenes
+
+    cfg = config_for("batch")
+    cfg.precond = precond
+    scenes = [sc for _, sc in batch_member_scenes(64, seed=7)]
+    merged = pa.Scene.merge(scenes)
+    cfg.max_contacts = 1 << 21
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_sync_to_host(False)
+    for k in range(3):
+        st = b.step()
+        it, ms = b.batch_stats(len(scenes))
+        print(f"step {k}: contacts {st['n_contacts']}, PCG iters per scene max {it.max()} median {int(np.median(it))}, "
+              f"solve ms per scene p50 {np.percentile(ms, 50):.3f} p99 {np.percentile(ms, 99):.3f}")
+        assert it.max() < cfg.admm_max_iter * cfg.global_max_iter
+    x, _ = b.get_state()
+    assert np.all(np.isfinite(x))
