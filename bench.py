@@ -143,6 +143,74 @@ def learned_leg(pa, args, scene, cfg, tracks, plain_stats, plain_value) -> dict:
     }
 
 
+def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) -> dict:
+    """The batched workload: `count` cloths (side ~ U{16..96}, drop ~ U(0.2,
+    0.6) m, sphere radius ~ U(0.1, 0.3) m, mu ~ U(0.1, 0.6); generator seed 0;
+    h = 1/60, 30 ADMM iterations) merged into one scene and stepped together.
+    Beside it, a sample of the same scenes stepped one at a time with the
+    single-scene backend (same settings and warm-up): their per-scene step
+    times give the p50/p99 and the one-at-a-time throughput."""
+    from parallel_admm_amd.configs import config_for
+    from parallel_admm_amd.warm_start import batch_member_scenes
+
+    members = list(batch_member_scenes(count, seed=0))
+    t0 = time.perf_counter()
+    merged = pa.Scene.merge([sc for _, sc in members])
+    cfg = config_for("batch")
+    cfg.max_contacts = 1 << 23
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_sync_to_host(False)
+    setup_s = time.perf_counter() - t0
+    for _ in range(warmup):
+        b.step()
+    print(f"[bench] batch of {count}: set-up {setup_s:.1f} s, warmup {warmup} steps done", file=sys.stderr, flush=True)
+    stats = []
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        stats.append(b.step())
+    wall = time.perf_counter() - t1
+    its, _ = b.batch_stats(count)
+    b.set_profiling(2)
+    b.step()
+    ph = b.phase_times()
+    b.set_profiling(0)
+    # one-at-a-time sample: every (count // solo_sample)-th scene
+    idx = list(range(0, count, max(1, count // solo_sample)))[:solo_sample]
+    solo_ms = []
+    for k in idx:
+        par, sc = members[k]
+        c = config_for("batch")
+        c.mu = par["mu"]
+        sb = pa.ADMMBackendHIP.from_scene(sc, c)
+        sb.set_sync_to_host(False)
+        for _ in range(warmup):
+            sb.step()
+        ts = time.perf_counter()
+        for _ in range(steps):
+            sb.step()
+        solo_ms.append(1000.0 * (time.perf_counter() - ts) / steps)
+    solo_ms = np.array(solo_ms)
+    sizes = np.array([par["side"] for par, _ in members])
+    out = {
+        "scenes": count, "vertices": merged.info["n_all"], "dynamic_vertices": merged.info["n_dyn"],
+        "side_min": int(sizes.min()), "side_max": int(sizes.max()), "generator_seed": 0,
+        "steps": steps, "warmup": warmup, "setup_s": setup_s,
+        "batched_steps_per_s": steps / wall, "scene_steps_per_s": count * steps / wall, "ms_per_batched_step": 1000.0 * wall / steps,
+        "per_scene_step_ms_in_batch": 1000.0 * wall / steps,
+        "phase_ms": dict(zip(("detection", "local", "contact", "rhs", "global", "other"), (float(v) for v in ph))),
+        "contacts_last": stats[-1]["n_contacts"],
+        "pcg_iters_to_own_tol_p50": float(np.percentile(its, 50)), "pcg_iters_to_own_tol_p99": float(np.percentile(its, 99)),
+        "solo_sample": len(idx), "solo_step_ms_p50": float(np.percentile(solo_ms, 50)), "solo_step_ms_p99": float(np.percentile(solo_ms, 99)),
+        "solo_scene_steps_per_s": float(1000.0 / solo_ms.mean()),
+        "batch_speedup_vs_one_at_a_time": float((count * steps / wall) / (1000.0 / solo_ms.mean())),
+        "global_solve": "AMG-PCG over the block-diagonal batch, stopped per scene",
+    }
+    print(f"[bench] batch of {count}: {out['batched_steps_per_s']:.2f} batched steps/s = {out['scene_steps_per_s']:.0f} scene-steps/s; "
+          f"one at a time {out['solo_scene_steps_per_s']:.0f} scene-steps/s (step ms p50 {out['solo_step_ms_p50']:.1f}, "
+          f"p99 {out['solo_step_ms_p99']:.1f}); phases {', '.join(f'{v:.1f}' for v in ph)}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +222,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned warm-start leg")
+    ap.add_argument("--batch-scenes", type=int, default=512, help="scenes in the batched leg (0 skips it)")
+    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-warmup", type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -219,6 +290,14 @@ def main() -> int:
 
     learned = None if args.no_learned else learned_leg(pa, args, scene, cfg, tracks, stats, args.steps / elapsed)
 
+    batch = None
+    if args.batch_scenes > 0:
+        try:
+            batch = batch_leg(pa, args.batch_scenes, args.batch_warmup, args.batch_steps)
+        except Exception as e:  # reported, not fatal to the headline line
+            batch = {"error": f"{type(e).__name__}: {e}"}
+            print(f"[bench] batched leg failed: {e}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         print(f"[bench] CPU oracle baseline (~{args.cpu_seconds:.0f} s) ...", file=sys.stderr, flush=True)
@@ -268,6 +347,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "learned": learned,
+        "batch": batch,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -76,6 +76,7 @@ struct Backend {
     DBuf<double> cg_state; // two slots of (rz[3], done[3]); [16..18] threshold, [19] iterations, [20..22] |b|^2
     int cg_blocks = 0;     // grid of every kernel that writes cg_part (grid-stride over rows)
     int a_lpr = 2;         // lanes per row for SpMVs with A
+    int rhs_lpr = 2;       // lanes per vertex for the D^T gather of the right-hand side
     int last_pcg_iters = 0; // iterations of the previous solve's first pass (sizes the first launch chunk)
     // AMG preconditioner (device copy of the hierarchy built at init)
     struct DevLevel {

@@ -651,17 +651,22 @@ __global__ void k_ws_log(int nc, float s, const float* __restrict__ feat, const 
 
 // b = bconst + sum_e coef_e (z - Ue)[row_e] + h^2 wc (h p + x0 - Uc), accumulated in FP64.
 // With want_res, also |sum coef (z - zprev)|^2 and |sum coef z|^2 into part[blk*6 + 4..5].
+// LPR lanes share a vertex (each takes every LPR-th entry of its D^T row
+// list) and combine with shuffles, so a vertex's dependent gathers of z and
+// Ue are spread over LPR lanes.
+template <int LPR>
 __global__ void __launch_bounds__(BLK) k_rhs(int n, float h, int contact, int want_res, const int32_t* __restrict__ dt_ptr,
                                              const int32_t* __restrict__ dt_row, const float* __restrict__ dt_coef,
                                              const float* __restrict__ z, const float* __restrict__ zprev, const float* __restrict__ Ue,
                                              const double* __restrict__ bconst, const float* __restrict__ wch2, const float* __restrict__ p,
                                              const float* __restrict__ x0, const float* __restrict__ Uc, double* __restrict__ b,
                                              double* __restrict__ part) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = t / LPR, sub = t % LPR;
     double acc[2] = {0, 0};
+    double s[3] = {0, 0, 0}, s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
     if (i < n) {
-        double s[3] = {0, 0, 0}, s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
-        for (int e = dt_ptr[i]; e < dt_ptr[i + 1]; e++) {
+        for (int e = dt_ptr[i] + sub; e < dt_ptr[i + 1]; e += LPR) {
             const int r = dt_row[e];
             const double cf = dt_coef[e];
 #pragma unroll
@@ -674,6 +679,18 @@ __global__ void __launch_bounds__(BLK) k_rhs(int n, float h, int contact, int wa
                 }
             }
         }
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            s[a] += __shfl_xor(s[a], o, LPR);
+            if (want_res) {
+                s1[a] += __shfl_xor(s1[a], o, LPR);
+                s2[a] += __shfl_xor(s2[a], o, LPR);
+            }
+        }
+    if (i < n && sub == 0) {
         const double wv = contact ? (double)wch2[i] : 0.0;
 #pragma unroll
         for (int a = 0; a < 3; a++) {
@@ -1824,7 +1841,8 @@ void Backend::assemble_and_upload(const double* x0h) {
     cg_part.alloc(9 * (size_t)cg_blocks);
     cg_state.alloc(24);
     int elem_blocks = grid_for(hc.n_tri, BLK) + grid_for(hc.n_tet, BLK) + grid_for(hc.n_bend, BLK) + grid_for(hc.n_pin, BLK);
-    res_blocks = std::max(std::max(elem_blocks, grid_for(n, BLK)), 1);
+    rhs_lpr = lanes_per_row((int64_t)dtrow.size(), n);
+    res_blocks = std::max(std::max(elem_blocks, grid_for((int64_t)n * rhs_lpr, BLK)), 1); // >= every grid writing res_part
     res_part.alloc(6 * (size_t)res_blocks);
     res_sum.alloc(6);
     // contact buffers sized by capacity
@@ -1947,8 +1965,12 @@ void Backend::contact_sweep_once() {
 }
 
 void Backend::rhs(bool want_res, bool) {
-    hipLaunchKernelGGL(k_rhs, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)cfg.dt, cfg.enable_contact, want_res ? 1 : 0,
-                       dt_ptr.p, dt_row.p, dt_coef.p, z.p, zprev.p, Ue.p, bconst.p, wch2.p, p.p, x0.p, Uc.p, b.p, res_part.p);
+    with_lpr(rhs_lpr, [&](auto c) {
+        constexpr int P = decltype(c)::value;
+        hipLaunchKernelGGL(k_rhs<P>, dim3(grid_for((int64_t)n * P, BLK)), dim3(BLK), 0, stream, n, (float)cfg.dt, cfg.enable_contact,
+                           want_res ? 1 : 0, dt_ptr.p, dt_row.p, dt_coef.p, z.p, zprev.p, Ue.p, bconst.p, wch2.p, p.p, x0.p, Uc.p, b.p,
+                           res_part.p);
+    });
     PA_LAUNCH_CHECK();
 }
 
