@@ -175,7 +175,7 @@ def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) ->
     ph = b.phase_times()
     b.set_profiling(0)
     # one-at-a-time sample: every (count // solo_sample)-th scene
-    idx = list(range(0, count, max(1, count // solo_sample)))[:solo_sample]
+    idx = list(range(0, count, max(1, count // solo_sample)))[:solo_sample] if solo_sample > 0 else []
     solo_ms = []
     for k in idx:
         par, sc = members[k]
@@ -189,7 +189,7 @@ def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) ->
         for _ in range(steps):
             sb.step()
         solo_ms.append(1000.0 * (time.perf_counter() - ts) / steps)
-    solo_ms = np.array(solo_ms)
+    solo_ms = np.array(solo_ms if solo_ms else [float("nan")])
     sizes = np.array([par["side"] for par, _ in members])
     out = {
         "scenes": count, "vertices": merged.info["n_all"], "dynamic_vertices": merged.info["n_dyn"],
@@ -216,7 +216,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cloth256", choices=["cloth256", "tetbox", "cloth32"])
+    ap.add_argument("--workload", default="cloth256", choices=["cloth256", "tetbox", "cloth32", "batch512"],
+                    help="batch512: only the batched leg (512 uneven cloths stepped together)")
     ap.add_argument("--tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -225,6 +226,7 @@ def main() -> int:
     ap.add_argument("--batch-scenes", type=int, default=512, help="scenes in the batched leg (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=20)
+    ap.add_argument("--batch-solo", type=int, default=16, help="scenes of the batch also stepped one at a time")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -235,6 +237,16 @@ def main() -> int:
 
     import parallel_admm_amd as pa
     from parallel_admm_amd.configs import SCENE_KIND, config_for, workload_tracks
+
+    if args.workload == "batch512":
+        bl = batch_leg(pa, args.batch_scenes or 512, args.batch_warmup, args.batch_steps, args.batch_solo)
+        print(json.dumps({
+            "metric": "scene steps/s of the batched workload (512 uneven cloths stepped together)",
+            "value": bl["scene_steps_per_s"], "unit": "scene-steps/s", "n_gpus": 1, "steps": args.batch_steps,
+            "warmup": args.batch_warmup, "ms_per_step": bl["ms_per_batched_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32", "data": "synthetic", "config": {"workload": "batch512", "parallelism": "single GPU"},
+            "batch": bl}), flush=True)
+        return 0
 
     scene = pa.Scene.generate(SCENE_KIND[args.workload])
     cfg = config_for(args.workload)
@@ -293,7 +305,7 @@ def main() -> int:
     batch = None
     if args.batch_scenes > 0:
         try:
-            batch = batch_leg(pa, args.batch_scenes, args.batch_warmup, args.batch_steps)
+            batch = batch_leg(pa, args.batch_scenes, args.batch_warmup, args.batch_steps, args.batch_solo)
         except Exception as e:  # reported, not fatal to the headline line
             batch = {"error": f"{type(e).__name__}: {e}"}
             print(f"[bench] batched leg failed: {e}", file=sys.stderr, flush=True)

@@ -5,16 +5,19 @@
 // Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
 // -> node boxes from leaf ranges -> 64-byte node records (both child boxes,
 // child ids and the last sorted leaf under each child, so a traversal step is
-// one dependent load) -> two traversal passes (count, then write into scanned
-// segments) with the FP64 predicate.
+// one dependent load) -> one traversal per query with the FP64 predicate,
+// accepted pairs appended to a staging list with their segment.
 //
-// Point-triangle: one lane per surface vertex; the vertex owns its segment and
-// sorts it by face id.  Edge-edge: one lane per sorted leaf k, visiting only
-// leaves after k, so each unordered pair is tested once; the pair is counted
-// and stored in the segment of its smaller edge id (integer atomics for the
-// per-segment counts and slots), and every segment is then sorted by the
-// larger id.  The list comes out in canonical order, bit-identical on every
-// run, with no global sort.
+// Point-triangle: one lane per surface vertex; its pairs form the vertex's
+// segment.  Edge-edge: one lane per sorted leaf k, visiting only leaves after
+// k, so each unordered pair is tested once; the pair belongs to the segment of
+// its smaller edge id.  Segment counts -> exclusive scan -> scatter of the
+// staged pairs into their segments -> per-segment sort by partner id: the list
+// comes out in canonical order, bit-identical on every run, with no global sort.
+//
+// Segmented mode (merged scenes): keys carry the scene id above a 21-bit
+// Morton code, so every scene is one subtree and queries start at their own
+// scene's subtree root.
 //
 // Reference: lbvh_f/lbvh_e::Construct (src/contact/lbvh/lbvh.cu:1468-1540),
 // _dcd_vf/_dcd_ee (:1694-1813); acceptance follows ProximalQuery
