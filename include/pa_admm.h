@@ -39,7 +39,9 @@ enum {
     PA_ERR_NO_DEVICE = 3, /* no GPU visible */
     PA_ERR_OVERFLOW = 4,  /* contact capacity exceeded (contacts were not truncated silently) */
     PA_ERR_NOT_SPD = 5,   /* system matrix not positive definite */
-    PA_ERR_GEOMETRY = 6   /* degenerate or inverted rest geometry */
+    PA_ERR_GEOMETRY = 6,  /* degenerate or inverted rest geometry */
+    PA_ERR_NUMERICAL = 7  /* a step produced non-finite positions or velocities; the state was rolled back
+                             to the end of the last good step (or init / reset) */
 };
 
 /* ---------------------------------------------------------------------------
@@ -267,6 +269,14 @@ pa_status pa_backend_set_static_state(pa_backend* b, const double* x_static, con
  * frame 0.  Call after pa_backend_init. */
 pa_status pa_backend_add_rigid_track(pa_backend* b, int32_t first, int32_t count, int32_t n_frames, const double* pos,
                                      const double* quat_xyzw, int32_t frames_per_step);
+
+/* Frame counters of the rigid tracks in registration order (*count = number
+ * of tracks; frames may be NULL).  Setting them (for resuming a checkpoint,
+ * after pa_backend_set_state has restored the positions) also re-places each
+ * track's last-placed positions at that frame, so the next step's static
+ * velocities equal an uninterrupted run's. */
+pa_status pa_backend_get_track_frames(pa_backend* b, int32_t* frames, int32_t* count);
+pa_status pa_backend_set_track_frames(pa_backend* b, const int32_t* frames, int32_t count);
 
 /* Convenience: runs the whole call order above for a scene (creates and
  * attaches its own detector when contact is enabled). */

@@ -23,7 +23,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libpa_admm.so")
 
-PA_OK, PA_ERR_INVALID, PA_ERR_HIP, PA_ERR_NO_DEVICE, PA_ERR_OVERFLOW, PA_ERR_NOT_SPD, PA_ERR_GEOMETRY = range(7)
+PA_OK, PA_ERR_INVALID, PA_ERR_HIP, PA_ERR_NO_DEVICE, PA_ERR_OVERFLOW, PA_ERR_NOT_SPD, PA_ERR_GEOMETRY, PA_ERR_NUMERICAL = range(8)
 
 # Every symbol declared in include/pa_admm.h.
 EXPORTS = [
@@ -42,7 +42,7 @@ EXPORTS = [
     "pa_backend_read_seed_log", "pa_backend_get_contact_duals", "pa_mlp_create", "pa_mlp_destroy", "pa_mlp_get_params",
     "pa_mlp_set_params", "pa_mlp_forward", "pa_mlp_gradients", "pa_mlp_adam_step", "pa_mlp_train", "pa_mlp_time",
     "pa_scene_merge", "pa_scene_get_segments", "pa_detector_set_segments", "pa_backend_batch_stats", "pa_backend_set_batch_tuning",
-    "pa_backend_get_batch_tuning",
+    "pa_backend_get_batch_tuning", "pa_backend_get_track_frames", "pa_backend_set_track_frames",
     "pa_last_error_message",
 ]
 
@@ -524,6 +524,17 @@ class ADMMBackendHIP:
             raise ValueError("pos and quat_xyzw need the same number of frames")
         self._err(lib().pa_backend_add_rigid_track(self._h, int(first), int(count), len(pos), _ptr(pos), _ptr(q),
                                                    int(frames_per_step)), "add_rigid_track")
+
+    def track_frames(self) -> np.ndarray:
+        cnt = C.c_int32()
+        self._err(lib().pa_backend_get_track_frames(self._h, None, C.byref(cnt)), "get_track_frames")
+        out = np.zeros(max(cnt.value, 1), np.int32)
+        self._err(lib().pa_backend_get_track_frames(self._h, _ptr(out), C.byref(cnt)), "get_track_frames")
+        return out[:cnt.value]
+
+    def set_track_frames(self, frames) -> None:
+        f = np.ascontiguousarray(frames, np.int32)
+        self._err(lib().pa_backend_set_track_frames(self._h, _ptr(f) if f.size else None, int(f.size)), "set_track_frames")
 
     def spmv_time(self, reps: int = 200) -> float:
         """Per-launch time (ms, HIP events) of `reps` back-to-back launches of the

@@ -128,6 +128,12 @@ struct Backend {
     DBuf<double> ws_err;            // judge partial sums
     DBuf<float> ws_consts;          // in_shift[10], in_scale[10], out_scale[2]
     DBuf<float> snap_x, snap_v, snap_ue, snap_uc; // step-start state for a cold re-run
+    DBuf<float> good_x, good_v, good_ue, good_uc; // last good state (init, reset, every finite step)
+    DBuf<uint32_t> bad_flag;
+    bool state_dirty = false; // set_state / set_duals / set_static_state since the last step
+    bool state_nonfinite();
+    void save_good_state();
+    void restore_good_state();
     int64_t log_cap = 0, log_rows = 0;
     DBuf<float> log_buf;
     bool seed_log = false;
@@ -192,6 +198,8 @@ struct Backend {
     // them at trans_k + R(q_k) rest; see pa_backend_add_rigid_track.
     void add_rigid_track(int first, int count, int frames, const double* pos, const double* quat_xyzw, int fps);
     void animate_tracks(bool place_first_frame);
+    void get_track_frames(int32_t* frames) const;
+    void set_track_frames(const int32_t* frames, int count);
     int global_solve(int max_iter, double tol);
     void pcg_iteration(int it, bool fused, int& pc, int& rc);
     hipGraphExec_t pcg_pair_graph(bool first_pair);

@@ -475,6 +475,17 @@ pa_status pa_backend_add_rigid_track(pa_backend* b, int32_t first, int32_t count
     BK(b->b->add_rigid_track(first, count, n_frames, pos, quat_xyzw, frames_per_step));
 }
 
+pa_status pa_backend_get_track_frames(pa_backend* b, int32_t* frames, int32_t* count) {
+    BK({
+        PA_CHECK(count, PA_ERR_INVALID, "null argument");
+        *count = (int32_t)b->b->tracks.size();
+        if (frames) b->b->get_track_frames(frames);
+    });
+}
+pa_status pa_backend_set_track_frames(pa_backend* b, const int32_t* frames, int32_t count) {
+    BK(PA_CHECK(frames || count == 0, PA_ERR_INVALID, "null argument"); b->b->set_track_frames(frames, count));
+}
+
 pa_status pa_backend_create_from_scene(const pa_scene* sc, const pa_admm_config* cfg, const pa_device_opts* o, pa_backend** out) {
     return guard(nullptr, [&] {
         PA_CHECK(sc && cfg && out, PA_ERR_INVALID, "null argument");

@@ -19,6 +19,8 @@
 // src/solver/contact_solver.cu:316-335).
 #include "pa_backend.hpp"
 
+#include <roctracer/roctx.h>
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -134,6 +136,13 @@ __global__ void k_begin_step(int n_all, int n, float h, float g, const uint8_t* 
         bconst[3 * i + 2] = mi * xt.z;
     }
     st3(p, i, vi);
+}
+
+// flag[0] |= 1 if any of x (3 n_all) or v (3 n_all) is not finite.
+__global__ void k_nonfinite(int64_t m, const float* __restrict__ x, const float* __restrict__ v, uint32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (!isfinite(x[i]) || !isfinite(v[i])) atomicOr(flag, 1u);
 }
 
 __global__ void k_end_step(int n, float h, const float* __restrict__ x, const float* __restrict__ x0, float* __restrict__ v) {
@@ -1346,12 +1355,14 @@ __global__ void k_animate_rigid(int first, int count, const double* __restrict__
     const double ny = ty + (ry + qw * Ty + (qz * Tx - qx * Tz));
     const double nz = tz + (rz + qw * Tz + (qx * Ty - qy * Tx));
     const int g = first + i;
-    x[3 * g] = (float)nx;
-    x[3 * g + 1] = (float)ny;
-    x[3 * g + 2] = (float)nz;
-    v[3 * g] = zero_v ? 0.f : (float)((nx - prev[3 * i]) * hinv);
-    v[3 * g + 1] = zero_v ? 0.f : (float)((ny - prev[3 * i + 1]) * hinv);
-    v[3 * g + 2] = zero_v ? 0.f : (float)((nz - prev[3 * i + 2]) * hinv);
+    if (x) { // without x and v only the placed positions are kept (resuming at a frame)
+        x[3 * g] = (float)nx;
+        x[3 * g + 1] = (float)ny;
+        x[3 * g + 2] = (float)nz;
+        v[3 * g] = zero_v ? 0.f : (float)((nx - prev[3 * i]) * hinv);
+        v[3 * g + 1] = zero_v ? 0.f : (float)((ny - prev[3 * i + 1]) * hinv);
+        v[3 * g + 2] = zero_v ? 0.f : (float)((nz - prev[3 * i + 2]) * hinv);
+    }
     prev[3 * i] = nx;
     prev[3 * i + 1] = ny;
     prev[3 * i + 2] = nz;
@@ -1486,7 +1497,10 @@ void Backend::init(const double* x0h, const double* massh, int n_all_, int n_dyn
     }
     assemble_and_upload(x0h);
     x_init_h.assign(x0h, x0h + 3 * (size_t)n_all);
+    bad_flag.alloc(1);
     initialized = true;
+    save_good_state();
+    PA_HIP(hipStreamSynchronize(stream));
 }
 
 // Registers a rigid track and places its vertices at frame 0 with zero
@@ -1548,6 +1562,30 @@ void Backend::animate_tracks(bool place_first_frame) {
                            t[0], t[1], t[2], q[0], q[1], q[2], q[3], hinv, zero_v ? 1 : 0, x.p, v.p);
         PA_LAUNCH_CHECK();
     }
+}
+
+// Frame counters of the tracks (registration order), for checkpoints.
+// Setting them also re-places each track's last-placed positions at that
+// frame (FP64, as the step that reached it did), so the next step's static
+// velocities are the ones an uninterrupted run computes.
+void Backend::get_track_frames(int32_t* frames) const {
+    for (size_t k = 0; k < tracks.size(); k++) frames[k] = tracks[k]->cur;
+}
+
+void Backend::set_track_frames(const int32_t* frames, int count) {
+    PA_CHECK(count == (int)tracks.size(), PA_ERR_INVALID, "one frame counter per track");
+    for (int k = 0; k < count; k++) PA_CHECK(frames[k] >= 1 && frames[k] <= tracks[k]->frames, PA_ERR_INVALID, "frame counter out of range");
+    for (int k = 0; k < count; k++) {
+        RigidTrack& T = *tracks[k];
+        T.cur = frames[k];
+        const int f = T.cur - 1;
+        const double* t = T.trans.data() + 3 * (size_t)f;
+        const double* q = T.quat.data() + 4 * (size_t)f;
+        hipLaunchKernelGGL(k_animate_rigid, dim3(grid_for(T.count, BLK)), dim3(BLK), 0, stream, T.first, T.count, T.rest.p, T.prev.p,
+                           t[0], t[1], t[2], q[0], q[1], q[2], q[3], 0.0, 0, (float*)nullptr, (float*)nullptr);
+        PA_LAUNCH_CHECK();
+    }
+    PA_HIP(hipStreamSynchronize(stream));
 }
 
 void Backend::assemble_and_upload(const double* x0h) {
@@ -2344,7 +2382,12 @@ void Backend::residual_read(double* rp, double* rd) {
     *rd = std::sqrt(s[4]) / std::max(1.0, std::sqrt(s[5]));
 }
 
-void Backend::mark_begin(int, hipEvent_t* e0) {
+// Phase ranges are also roctx ranges (pa.detection, pa.local, ...), visible
+// to rocprofv3 --marker-trace; without a tracer attached they cost a call.
+static const char* const kPhaseNames[6] = {"pa.detection", "pa.local", "pa.contact", "pa.rhs", "pa.global", "pa.other"};
+
+void Backend::mark_begin(int phase, hipEvent_t* e0) {
+    roctxRangePushA(kPhaseNames[phase]);
     *e0 = nullptr;
     if (!(profiling & 2)) return;
     *e0 = next_event();
@@ -2352,6 +2395,7 @@ void Backend::mark_begin(int, hipEvent_t* e0) {
 }
 
 void Backend::mark_end(int phase, hipEvent_t e0) {
+    roctxRangePop();
     if (!(profiling & 2)) return;
     hipEvent_t e1 = next_event();
     PA_HIP(hipEventRecord(e1, stream));
@@ -2517,6 +2561,43 @@ void Backend::ws_window_begin() {
     }
 }
 
+bool Backend::state_nonfinite() {
+    fill<uint32_t>(bad_flag.p, 0u, 1, stream);
+    hipLaunchKernelGGL(k_nonfinite, dim3(grid_for(3 * (int64_t)n_all, BLK)), dim3(BLK), 0, stream, 3 * (int64_t)n_all, (const float*)x.p,
+                       (const float*)v.p, bad_flag.p);
+    PA_LAUNCH_CHECK();
+    uint32_t bad = 0;
+    PA_HIP(hipMemcpyAsync(&bad, bad_flag.p, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    PA_HIP(hipStreamSynchronize(stream));
+    return bad != 0;
+}
+
+// The last good state (x, v, Ue, Uc): kept at init, reset and after every
+// step whose state is finite; a non-finite step returns to it.
+void Backend::save_good_state() {
+    auto keep = [&](DBuf<float>& dst, const DBuf<float>& src, size_t count) {
+        dst.reserve(count);
+        if (count) PA_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(float) * count, hipMemcpyDeviceToDevice, stream));
+    };
+    keep(good_x, x, 3 * (size_t)n_all);
+    keep(good_v, v, 3 * (size_t)n_all);
+    keep(good_ue, Ue, 3 * (size_t)m);
+    keep(good_uc, Uc, 3 * (size_t)n);
+}
+
+void Backend::restore_good_state() {
+    auto back = [&](DBuf<float>& dst, const DBuf<float>& src, size_t count) {
+        if (count) PA_HIP(hipMemcpyAsync(dst.p, src.p, sizeof(float) * count, hipMemcpyDeviceToDevice, stream));
+    };
+    back(x, good_x, 3 * (size_t)n_all);
+    back(v, good_v, 3 * (size_t)n_all);
+    back(Ue, good_ue, 3 * (size_t)m);
+    back(Uc, good_uc, 3 * (size_t)n);
+    ws_forget();
+    PA_HIP(hipStreamSynchronize(stream));
+    if (sync_to_host) get_state(x_host.data(), v_host.data());
+}
+
 // On failure inside the step the positions roll back to the start of the
 // step (velocities are untouched until the epilogue); the duals keep the
 // iterations already applied.
@@ -2525,7 +2606,7 @@ void Backend::step(pa_step_stats* st) {
     try {
         step_impl(st);
     } catch (const Error& err) {
-        if (err.code != PA_ERR_HIP) {
+        if (err.code != PA_ERR_HIP && err.code != PA_ERR_NUMERICAL) { // a numerical failure restored the last good state
             (void)hipMemcpyAsync(x.p, x0.p, sizeof(float) * 3 * (size_t)n_all, hipMemcpyDeviceToDevice, stream);
             (void)hipStreamSynchronize(stream);
         }
@@ -2541,6 +2622,13 @@ void Backend::step_impl(pa_step_stats* st) {
     const bool contact = cfg.enable_contact != 0;
     PA_CHECK(!contact || det, PA_ERR_INVALID, "contact enabled but no detector attached");
     auto t0 = std::chrono::steady_clock::now();
+    if (state_dirty) { // state set from outside since the last step: check it before detection uses it
+        state_dirty = false;
+        if (state_nonfinite()) {
+            restore_good_state();
+            throw Error(PA_ERR_NUMERICAL, "non-finite positions or velocities given to the step; state rolled back to the last good step");
+        }
+    }
     pa_step_stats S{};
     if (!tracks.empty()) animate_tracks(false);
     const double hd = cfg.dt;
@@ -2611,6 +2699,12 @@ void Backend::step_impl(pa_step_stats* st) {
     hipLaunchKernelGGL(k_end_step, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (float)hd, x.p, x0.p, v.p);
     PA_LAUNCH_CHECK();
     mark_end(5, e);
+    // non-finite positions or velocities: back to the last good state
+    if (state_nonfinite()) {
+        restore_good_state();
+        throw Error(PA_ERR_NUMERICAL, "non-finite positions or velocities after the step; state rolled back to the last good step");
+    }
+    save_good_state();
     if (sync_to_host) {
         std::vector<float> xf(3 * (size_t)n_all), vf(3 * (size_t)n_all);
         x.download(xf.data(), xf.size(), stream);
@@ -2716,6 +2810,7 @@ void Backend::reset(const double* x0h) {
     PA_HIP(hipStreamSynchronize(stream));
     ws_forget();
     x_host.assign(x0h, x0h + 3 * (size_t)n_all);
+    save_good_state();
     std::fill(v_host.begin(), v_host.end(), 0.0);
     if (!tracks.empty()) {
         animate_tracks(true);
@@ -2736,6 +2831,7 @@ void Backend::get_state(double* xo, double* vo) {
 }
 
 void Backend::set_state(const double* xi, const double* vi) {
+    state_dirty = true;
     PA_CHECK(initialized, PA_ERR_INVALID, "set_state before init");
     std::vector<float> f(3 * (size_t)n_all);
     if (xi) {
@@ -2763,6 +2859,7 @@ void Backend::get_duals(double* ue, double* uc) {
 }
 
 void Backend::set_duals(const double* ue, const double* uc) {
+    state_dirty = true;
     PA_CHECK(initialized, PA_ERR_INVALID, "set_duals before init");
     if (ue) {
         std::vector<float> a(ue, ue + 3 * (size_t)m);
@@ -2777,6 +2874,7 @@ void Backend::set_duals(const double* ue, const double* uc) {
 }
 
 void Backend::set_static_state(const double* xs, const double* vs) {
+    state_dirty = true;
     PA_CHECK(initialized, PA_ERR_INVALID, "set_static_state before init");
     const size_t ns = 3 * (size_t)(n_all - n);
     if (ns == 0) return;

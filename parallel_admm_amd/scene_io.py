@@ -367,13 +367,28 @@ def read_frame(path: str) -> np.ndarray:
     return np.frombuffer(body[: 12 * nv], "<f4").reshape(nv, 3).astype(np.float64)
 
 
+EXTRA_METRICS = ("admm_iters", "converged", "r_primal", "r_dual", "pcg_iters", "detections", "ws_seeded", "ws_fallback")
+
+
 def write_step_metrics_csv(path: str, metrics) -> None:
-    """step_metrics.csv: header frame,step_ms,n_contacts (solver.h:38-45)."""
+    """step_metrics.csv: header frame,step_ms,n_contacts (solver.h:38-45).
+    Rows of more than three values add the columns of EXTRA_METRICS (ADMM
+    iterations, convergence, final residuals, PCG iterations, detections,
+    warm-start use) after the reference's three, so readers of the
+    reference's file keep working."""
+    metrics = list(metrics)
+    extra = max((len(m) for m in metrics), default=3) - 3
     with open(path, "w", newline="") as f:
         w = csv.writer(f)
-        w.writerow(["frame", "step_ms", "n_contacts"])
-        for frame, ms, nc in metrics:
-            w.writerow([int(frame), float(ms), int(nc)])
+        w.writerow(["frame", "step_ms", "n_contacts"] + list(EXTRA_METRICS[:extra]))
+        for m in metrics:
+            w.writerow([int(m[0]), float(m[1]), int(m[2])] + list(m[3:]))
+
+
+def step_metrics_row(step: int, st: dict) -> tuple:
+    """One extended step_metrics.csv row from a pa_step_stats dict."""
+    return (step, st["step_ms"], st["n_contacts"], st["iterations"], st["converged"], st["r_primal"], st["r_dual"],
+            st["global_iters"], st["detections"], st["ws_seeded"], st["ws_fallback"])
 
 
 def _load_frames(run_dir: str) -> dict:
@@ -429,7 +444,7 @@ def run_scene(spec: SceneSpec, out_dir: str, frames: int | None = None, export: 
     """The reference's frame loop without the GUI (main.cpp:29-94): per frame
     `sub_step` solver steps, then the frame is exported as
     <scene>_<frame>.obj|.ply; step_metrics.csv gets one row per solver step.
-    Returns the metrics rows (frame, step_ms, n_contacts)."""
+    Returns the metrics rows (frame, step_ms, n_contacts, then EXTRA_METRICS)."""
     from . import ADMMBackendHIP
 
     b = ADMMBackendHIP.from_scene(spec.scene, spec.config, device=device)
@@ -443,7 +458,7 @@ def run_scene(spec: SceneSpec, out_dir: str, frames: int | None = None, export: 
     for fr in range(n_frames):
         for _ in range(spec.sub_step):
             st = b.step()
-            metrics.append((step, st["step_ms"], st["n_contacts"]))
+            metrics.append(step_metrics_row(step, st))
             step += 1
         if export:
             export_frame(out_dir, spec.scene_name, fr, b.get_state()[0], tris, binary)

@@ -313,3 +313,77 @@ def test_step_overflow_rolls_back(pa):
         b.step()
     assert e.value.code == pa.PA_ERR_OVERFLOW
     np.testing.assert_array_equal(b.get_state()[0], before)
+
+
+def test_nonfinite_state_rolls_back(pa):
+    """NaN injected into x (or Inf into v) from the host: the next step
+    reports PA_ERR_NUMERICAL and the state is the one after the last good
+    step, bit for bit; stepping then continues as if nothing happened."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    sc = pa.Scene.generate(4, side=24, radius=0.2, drop=0.002)
+    ref = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    for _ in range(3):
+        ref.step()
+        b.step()
+    good_x, good_v = b.get_state()
+    for field in ("x", "v"):
+        x, v = b.get_state()
+        if field == "x":
+            x[5, 1] = np.nan
+        else:
+            v[7, 0] = np.inf
+        b.set_state(x, v)
+        with pytest.raises(pa.PAError) as e:
+            b.step()
+        assert e.value.code == pa.PA_ERR_NUMERICAL
+        xr, vr = b.get_state()
+        np.testing.assert_array_equal(xr, good_x)
+        np.testing.assert_array_equal(vr, good_v)
+    for _ in range(2):
+        ref.step()
+        b.step()
+    np.testing.assert_array_equal(b.get_state()[0], ref.get_state()[0])
+
+
+@pytest.mark.parametrize("tracked", [False, True], ids=["plain", "rotating-obstacle"])
+def test_checkpoint_resume_is_bit_exact(pa, tmp_path, tracked):
+    """Six steps straight equal three steps, a checkpoint, a fresh backend
+    loading it, and three more steps (also with a rigid track, whose frame
+    counter and last placed positions are restored)."""
+    from parallel_admm_amd import checkpoint
+    from parallel_admm_amd.configs import config_for, rotation_track
+
+    cfg = config_for("cloth32")
+    sc = pa.Scene.generate(4, side=24, radius=0.2, drop=0.002)
+    n, n_all = sc.info["n_dyn"], sc.info["n_all"]
+    pos, quat = rotation_track((0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 3.0, cfg.dt, 64)
+
+    def fresh():
+        b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+        if tracked:
+            b.add_rigid_track(n, n_all - n, pos, quat)
+        return b
+
+    straight = fresh()
+    for _ in range(6):
+        straight.step()
+    first = fresh()
+    for _ in range(3):
+        first.step()
+    path = str(tmp_path / "ck.npz")
+    checkpoint.save(first, path, 3, cfg)
+    resumed = fresh()
+    assert checkpoint.load(resumed, path, cfg) == 3
+    for _ in range(3):
+        resumed.step()
+    xs, vs = straight.get_state()
+    xr, vr = resumed.get_state()
+    np.testing.assert_array_equal(xr, xs)
+    np.testing.assert_array_equal(vr, vs)
+    other = config_for("cloth32")
+    other.mu = 0.31
+    with pytest.raises(ValueError):
+        checkpoint.load(fresh(), path, other)

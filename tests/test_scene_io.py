@@ -213,6 +213,14 @@ def test_export_roundtrip_and_metrics(tmp_path):
     np.testing.assert_array_equal(sio.read_obj(p1)[1], f)
     sio.write_step_metrics_csv(str(tmp_path / "step_metrics.csv"), [(0, 1.5, 3), (1, 2.5, 4)])
     assert (tmp_path / "step_metrics.csv").read_text().splitlines() == ["frame,step_ms,n_contacts", "0,1.5,3", "1,2.5,4"]
+    # extended rows keep the reference's three columns first
+    st = dict(step_ms=2.0, n_contacts=5, iterations=7, converged=1, r_primal=1e-5, r_dual=2e-5, global_iters=30, detections=2,
+              ws_seeded=0, ws_fallback=0)
+    sio.write_step_metrics_csv(str(tmp_path / "m2.csv"), [sio.step_metrics_row(3, st)])
+    lines = (tmp_path / "m2.csv").read_text().splitlines()
+    assert lines[0] == "frame,step_ms,n_contacts," + ",".join(sio.EXTRA_METRICS)
+    assert lines[1].startswith("3,2.0,5,7,1,")
+    assert sio._load_metrics(str(tmp_path)) == {0: (1.5, 3), 1: (2.5, 4)}
 
 
 def test_compare_runs_known_differences(tmp_path):
@@ -281,7 +289,7 @@ def test_run_scene_exports_frames_and_metrics(pa, tmp_path):
         m = sio.run_scene(spec, str(out), frames=4)
         assert len(m) == 4 * spec.sub_step
         assert sorted(p.name for p in out.glob("*.obj")) == [f"unit_{k:05d}.obj" for k in range(4)]
-        assert (out / "step_metrics.csv").read_text().startswith("frame,step_ms,n_contacts\n")
+        assert (out / "step_metrics.csv").read_text().startswith("frame,step_ms,n_contacts," + ",".join(sio.EXTRA_METRICS) + "\n")
         runs.append(m)
     rows = sio.compare_runs(str(tmp_path / "r1"), str(tmp_path / "r2"), str(tmp_path / "frames.csv"))
     assert all(r[2] == 0.0 for r in rows)
