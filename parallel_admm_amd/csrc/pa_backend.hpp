@@ -35,6 +35,8 @@ struct Backend {
     int device = 0;
     int verbose = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;                       // the elastic local step beside the contact phase
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     std::string last_error;
 
     pa_admm_config cfg{};
@@ -188,7 +190,7 @@ struct Backend {
     void set_static_state(const double* xs, const double* vs);
 
     // sub-phases (also used by step)
-    void local_step(bool first_iter);
+    void local_step(bool first_iter, hipStream_t s = nullptr);
     void contact_setup(float eps, float gamma, float hinv);
     void contact_project(bool first_iter);
     void contact_sweep_once();

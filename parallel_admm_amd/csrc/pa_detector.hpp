@@ -49,6 +49,11 @@ struct Detector {
     DBuf<uint32_t> qfill;           // nv + ne: slots taken per segment by the scatter
     DBuf<uint32_t> total;           // [0] contacts found (segment counts), [1] stack overflow flag, [2] staged contacts
     PrimScratch scratch;
+    // The edge tree is built and queried on a second stream, concurrently
+    // with the triangle tree (own sort scratch; joined before the scan).
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    PrimScratch scratch_side;
     // Segmented mode (batched scenes): vertex v belongs to segment vseg[v];
     // pairs are only sought inside a segment.  Keys are (segment << 21 |
     // 21-bit Morton code), so each segment's leaves form one subtree of the
@@ -73,7 +78,7 @@ struct Detector {
     void set_stream(hipStream_t s) { stream = s; }
 
   private:
-    void build_tree(Tree& T, const float* x, bool faces);
+    void build_tree(Tree& T, const float* x, bool faces, hipStream_t s, PrimScratch& scr);
 };
 
 } // namespace pa

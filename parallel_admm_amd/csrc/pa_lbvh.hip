@@ -530,6 +530,9 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
         PA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
         stream = own_stream;
     }
+    PA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking)); // default priority
+    PA_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    PA_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     for (int i = 0; i < nv; i++) PA_CHECK(sv[i] >= 0 && sv[i] < n_all, PA_ERR_INVALID, "surface vertex out of range");
     for (int i = 0; i < 3 * nf; i++) PA_CHECK(t[i] >= 0 && t[i] < n_all, PA_ERR_INVALID, "surface triangle index out of range");
     for (int i = 0; i < 2 * ne; i++) PA_CHECK(e[i] >= 0 && e[i] < n_all, PA_ERR_INVALID, "surface edge index out of range");
@@ -581,13 +584,19 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
 }
 
 Detector::~Detector() {
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (own_stream) {
         (void)hipStreamSynchronize(own_stream);
         (void)hipStreamDestroy(own_stream);
     }
 }
 
-void Detector::build_tree(Tree& T, const float* x, bool faces) {
+void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t stream, PrimScratch& scratch) {
     const int n = T.n;
     if (n == 0) return;
     hipLaunchKernelGGL(k_prim_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, x, faces ? tris.p : edges.p, faces ? 3 : 2, n,
@@ -661,15 +670,18 @@ void Detector::set_segments(const int32_t* vs, int count) {
 }
 
 int Detector::detect(const float* x) {
-    build_tree(ft, x, true);
-    build_tree(et, x, false);
+    // the edge tree (build, query) runs on the side stream beside the triangle tree
     fill<uint32_t>(total.p, 0u, 3, stream);
+    if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
+    PA_HIP(hipEventRecord(ev_fork, stream));
+    PA_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+    build_tree(ft, x, true, stream, scratch);
+    build_tree(et, x, false, side, scratch_side);
     const float gap = (float)(dhat * 1.001);
     QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
     QueryOut st{stg.type.p, stg.prim0.p, stg.prim1.p, stg.v.p, stg.bary.p, stg.nrm.p, stg.dist.p, stg.cap};
     // one traversal per query: pairs to the staging list (total[2] counts
-    // them), per-segment counts to qcount
-    if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
+    // them, atomically from both streams), per-segment counts to qcount
     if (nv > 0) {
         hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, x, surf_verts.p, nv, tris.p, n_dyn, nf, ft.rec.p,
                            ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
@@ -677,11 +689,13 @@ int Detector::detect(const float* x) {
         PA_LAUNCH_CHECK();
     }
     if (ne > 0) {
-        hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, stream, x, edges.p, ne, n_dyn, et.rec.p, et.ids.p, dhat,
+        hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, side, x, edges.p, ne, n_dyn, et.rec.p, et.ids.p, dhat,
                            gap, qcount.p + nv, nv, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
                            nseg > 0 ? (const int32_t*)et.seg_root.p : nullptr);
         PA_LAUNCH_CHECK();
     }
+    PA_HIP(hipEventRecord(ev_join, side));
+    PA_HIP(hipStreamWaitEvent(stream, ev_join, 0));
     scan_exclusive(qcount.p, qoffset.p, (int64_t)nv + ne, total.p, scratch, stream);
     uint32_t h[3];
     PA_HIP(hipMemcpyAsync(h, total.p, sizeof(h), hipMemcpyDeviceToHost, stream));

@@ -1382,6 +1382,9 @@ Backend::Backend(const pa_device_opts* o) {
     PA_CHECK(device >= 0 && device < count, PA_ERR_INVALID, "device ordinal out of range");
     PA_HIP(hipSetDevice(device));
     PA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    PA_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking)); // default priority, as `stream`
+    PA_HIP(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    PA_HIP(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     pa_admm_config_default(&cfg);
 }
 
@@ -1394,6 +1397,12 @@ Backend::~Backend() {
     if (det && det != own_det.get()) det->stream = det->own_stream;
     own_det.reset();
     ws_mlp.reset(); // uses the stream
+    if (side) {
+        (void)hipStreamSynchronize(side);
+        (void)hipStreamDestroy(side);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -1941,11 +1950,11 @@ static ElemArgs make_elem_args(const Backend& B) {
 
 static int elem_grid(const ElemArgs& a, int n_pin) { return a.blk_bend + grid_for(n_pin, BLK); }
 
-void Backend::local_step(bool first_iter) {
+void Backend::local_step(bool first_iter, hipStream_t s) {
     ElemArgs a = make_elem_args(*this);
     int g = elem_grid(a, hc.n_pin);
     if (g == 0) return;
-    hipLaunchKernelGGL(k_local, dim3(g), dim3(BLK), 0, stream, a, first_iter ? 1 : 0, x.p, Ue.p, z.p, zprev.p);
+    hipLaunchKernelGGL(k_local, dim3(g), dim3(BLK), 0, s ? s : stream, a, first_iter ? 1 : 0, x.p, Ue.p, z.p, zprev.p);
     PA_LAUNCH_CHECK();
 }
 
@@ -2740,28 +2749,51 @@ int Backend::admm_loop(pa_step_stats& S, float eps, float gam_, float hinv) {
     nc = 0;
     double best = std::numeric_limits<double>::infinity();
     int it = 0;
+    // The elastic local step (z from x, Ue) and the contact phase (detection,
+    // projection: p from x, x0, Uc) are independent until the right-hand
+    // side: the local step runs on the side stream beside them.  Phase
+    // timing (profiling bit 1) keeps them in sequence.
+    const bool overlap = !(profiling & 2);
+    auto join = [&]() {
+        PA_HIP(hipEventRecord(ev_join, side));
+        PA_HIP(hipStreamWaitEvent(stream, ev_join, 0));
+    };
     for (; it < cfg.admm_max_iter; it++) {
+        if (overlap) {
+            PA_HIP(hipEventRecord(ev_fork, stream));
+            PA_HIP(hipStreamWaitEvent(side, ev_fork, 0));
+            local_step(it == 0, side);
+        }
         if (contact && it % cfg.dcd_interval == 0) {
             mark_begin(0, &e);
             ws_window_end();
             if (ws_restart) {
                 mark_end(0, e);
+                if (overlap) join();
                 break;
             }
-            det->detect(x.p);
+            try {
+                det->detect(x.p);
+            } catch (...) {
+                if (overlap) join(); // the rollback writes x, which the side stream reads
+                throw;
+            }
             contact_setup(eps, gam_, hinv);
             ws_window_begin();
             S.detections++;
             mark_end(0, e);
         }
-        mark_begin(1, &e);
-        local_step(it == 0);
-        mark_end(1, e);
+        if (!overlap) {
+            mark_begin(1, &e);
+            local_step(it == 0);
+            mark_end(1, e);
+        }
         if (contact) {
             mark_begin(2, &e);
             contact_project(it == 0);
             mark_end(2, e);
         }
+        if (overlap) join();
         const bool check = cfg.residual_tol > 0 && ((it + 1) % cfg.residual_interval == 0);
         mark_begin(3, &e);
         if (check) fill<double>(res_part.p, 0.0, 6 * (int64_t)res_blocks, stream);

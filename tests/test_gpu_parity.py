@@ -387,3 +387,33 @@ def test_checkpoint_resume_is_bit_exact(pa, tmp_path, tracked):
     other.mu = 0.31
     with pytest.raises(ValueError):
         checkpoint.load(fresh(), path, other)
+
+
+def test_degenerate_systems(pa):
+    """k = 0 (no stretch or bending stiffness) with every vertex pinned: the
+    system stays SPD through the mass term and the step holds every vertex at
+    its pin.  A zero-density cloth (zero mass) is refused before any solve."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.enable_contact = False
+    side = 6
+    g = np.arange(side)
+    x = np.stack([np.repeat(g, side) * 0.01, np.full(side * side, 0.5), np.tile(g, side) * 0.01], axis=1)
+    tris = []
+    for i in range(side - 1):
+        for j in range(side - 1):
+            a, b, c, d = i * side + j, (i + 1) * side + j, (i + 1) * side + j + 1, i * side + j + 1
+            tris += [[a, b, c], [a, c, d]]
+    tris = np.array(tris, np.int32)
+    n = side * side
+    sc = pa.Scene.from_mesh(x, n, np.ones(n), tris=tris, tri_k=np.zeros(len(tris)), bend_k=np.zeros(n), pins=np.arange(n))
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    for _ in range(3):
+        b.step()
+    xg, vg = b.get_state()
+    assert np.all(np.isfinite(xg)) and np.abs(xg - x).max() < 1e-6
+    with pytest.raises(pa.PAError) as e:
+        sc0 = pa.Scene.from_mesh(x, n, np.zeros(n), tris=tris, tri_k=np.ones(len(tris)), bend_k=np.zeros(n))
+        pa.ADMMBackendHIP.from_scene(sc0, cfg)
+    assert e.value.code in (pa.PA_ERR_INVALID, pa.PA_ERR_GEOMETRY)
