@@ -68,17 +68,22 @@ struct Detector {
              double dhat, hipStream_t s);
     ~Detector();
     // Runs detection on device positions x (3 n_all floats, on this->stream).
+    // With refit the trees keep the sorted order and topology of the last
+    // build and only their boxes are recomputed from x; the contact list does
+    // not depend on the tree, only the traversal's cost does.
     // Returns the number of contacts; the sorted list is complete once the
     // work queued on this->stream has run (the last kernels are queued after
     // the host reads the count).  Sets found_total and throws PA_ERR_OVERFLOW
     // (list unspecified) if more than `cap` contacts were found.
-    int detect(const float* x);
+    int detect(const float* x, bool refit = false);
     // Enables segmented mode: vseg (n_all host ints) in [0, nseg), nseg <= 2048.
     void set_segments(const int32_t* vseg_host, int nseg);
     void set_stream(hipStream_t s) { stream = s; }
 
   private:
-    void build_tree(Tree& T, const float* x, bool faces, hipStream_t s, PrimScratch& scr);
+    void build_tree(Tree& T, const float* x, bool faces, hipStream_t s, PrimScratch& scr, bool refit);
+    bool built = false; // trees built at least once (a refit needs their order and topology)
+    void node_boxes(Tree& T, hipStream_t s);
 };
 
 } // namespace pa

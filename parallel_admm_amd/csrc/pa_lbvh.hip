@@ -596,12 +596,18 @@ Detector::~Detector() {
     }
 }
 
-void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t stream, PrimScratch& scratch) {
+void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t stream, PrimScratch& scratch, bool refit) {
     const int n = T.n;
     if (n == 0) return;
     hipLaunchKernelGGL(k_prim_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, x, faces ? tris.p : edges.p, faces ? 3 : 2, n,
                        T.prim_box.p);
     PA_LAUNCH_CHECK();
+    if (refit) { // same sorted order and Karras topology, new boxes
+        hipLaunchKernelGGL(k_leaf_boxes, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, T.prim_box.p, T.ids.p, n, T.node_box.p);
+        PA_LAUNCH_CHECK();
+        if (n > 1) node_boxes(T, stream);
+        return;
+    }
     const int nb = std::min(256, grid_for(n, BLK));
     hipLaunchKernelGGL(k_box_union, dim3(nb), dim3(BLK), 0, stream, T.prim_box.p, n, T.partial.p);
     PA_LAUNCH_CHECK();
@@ -622,18 +628,7 @@ void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t strea
     if (n > 1) {
         hipLaunchKernelGGL(k_karras, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, T.keys.p, T.ids.p, n, T.child.p, T.range.p);
         PA_LAUNCH_CHECK();
-        const float4* leaves = T.node_box.p + 2 * (size_t)(n - 1);
-        const int n64 = (n + 63) / 64, n4k = (n + 4095) / 4096;
-        hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n64 * 64, BLK)), dim3(BLK), 0, stream, leaves, n, T.chunk64.p, n64);
-        PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n4k * 64, BLK)), dim3(BLK), 0, stream, T.chunk64.p, n64, T.chunk4k.p, n4k);
-        PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.range.p, leaves, T.chunk64.p, T.chunk4k.p,
-                           T.node_box.p);
-        PA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_node_records, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.child.p, T.range.p, T.node_box.p,
-                           T.rec.p);
-        PA_LAUNCH_CHECK();
+        node_boxes(T, stream);
     }
     if (nseg > 0) {
         fill<int32_t>(T.seg_first.p, -1, nseg, stream);
@@ -646,6 +641,24 @@ void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t strea
                            (const int32_t*)T.seg_first.p, (const int32_t*)T.seg_last.p, T.seg_root.p);
         PA_LAUNCH_CHECK();
     }
+}
+
+// Internal node boxes from the leaf boxes (chunk unions, range unions), then
+// the 64-byte node records.
+void Detector::node_boxes(Tree& T, hipStream_t stream) {
+    const int n = T.n;
+    const float4* leaves = T.node_box.p + 2 * (size_t)(n - 1);
+    const int n64 = (n + 63) / 64, n4k = (n + 4095) / 4096;
+    hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n64 * 64, BLK)), dim3(BLK), 0, stream, leaves, n, T.chunk64.p, n64);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_chunk_union, dim3(grid_for((int64_t)n4k * 64, BLK)), dim3(BLK), 0, stream, T.chunk64.p, n64, T.chunk4k.p, n4k);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_node_boxes, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.range.p, leaves, T.chunk64.p, T.chunk4k.p,
+                       T.node_box.p);
+    PA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for(n - 1, BLK)), dim3(BLK), 0, stream, n, T.child.p, T.range.p, T.node_box.p,
+                       T.rec.p);
+    PA_LAUNCH_CHECK();
 }
 
 void Detector::set_segments(const int32_t* vs, int count) {
@@ -662,6 +675,7 @@ void Detector::set_segments(const int32_t* vs, int count) {
     vseg.alloc(n_all);
     vseg.upload(vs, n_all, stream);
     nseg = count;
+    built = false; // keys change: the next detection rebuilds
     for (Tree* T : {&ft, &et}) {
         T->seg_first.alloc(count);
         T->seg_last.alloc(count);
@@ -669,14 +683,16 @@ void Detector::set_segments(const int32_t* vs, int count) {
     }
 }
 
-int Detector::detect(const float* x) {
+int Detector::detect(const float* x, bool refit) {
+    refit = refit && built;
     // the edge tree (build, query) runs on the side stream beside the triangle tree
     fill<uint32_t>(total.p, 0u, 3, stream);
     if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
     PA_HIP(hipEventRecord(ev_fork, stream));
     PA_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-    build_tree(ft, x, true, stream, scratch);
-    build_tree(et, x, false, side, scratch_side);
+    build_tree(ft, x, true, stream, scratch, refit);
+    build_tree(et, x, false, side, scratch_side, refit);
+    built = true;
     const float gap = (float)(dhat * 1.001);
     QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
     QueryOut st{stg.type.p, stg.prim0.p, stg.prim1.p, stg.v.p, stg.bary.p, stg.nrm.p, stg.dist.p, stg.cap};

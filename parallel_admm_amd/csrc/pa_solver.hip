@@ -520,12 +520,27 @@ __global__ void k_contact_du(int nc, float mu, const int4* __restrict__ cv, cons
     rc[c] = make_float4(r.x + d.x, r.y + d.y, r.z + d.z, 0.f);
 }
 
-__global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent,
-                                const float4* __restrict__ G, const float4* __restrict__ du) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (vptr[i] == vptr[i + 1]) return;
+// p_v += sum Gamma du over the contacts of v, for the *count dynamic vertices
+// that have contacts (cverts, ascending).
+__global__ void k_vertex_gather(const uint32_t* __restrict__ count, const uint32_t* __restrict__ cverts, float* __restrict__ p,
+                                const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent, const float4* __restrict__ G,
+                                const float4* __restrict__ du) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int)*count) return;
+    const int i = (int)cverts[t];
     st3(p, i, ld3(p, i) + gather_contacts(i, vptr, vent, G, du));
+}
+
+// flag[i] = 1 if dynamic vertex i has contacts
+__global__ void k_contact_flags(int n, const uint32_t* __restrict__ vptr, uint32_t* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = vptr[i + 1] > vptr[i] ? 1u : 0u;
+}
+
+// cverts[pos[i]] = i for flagged vertices: the vertices with contacts in ascending order
+__global__ void k_contact_compact(int n, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ pos, uint32_t* __restrict__ cverts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && vptr[i + 1] > vptr[i]) cverts[pos[i]] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1899,6 +1914,8 @@ void Backend::assemble_and_upload(const double* x0h) {
     rc.alloc(cap);
     du.alloc(cap);
     vcnt.alloc(n_all);
+    cverts.alloc(std::max(n, 1));
+    cvcount.alloc(1);
     vptr.alloc((size_t)n + 1);
     vfill.alloc(n);
     vent.alloc(4 * (size_t)cap);
@@ -1991,6 +2008,13 @@ void Backend::contact_setup(float eps, float gamma_, float hinv) {
         hipLaunchKernelGGL(k_contact_scc, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, eps, gamma_, hinv, det->c.v.p,
                            det->c.bary.p, det->c.nrm.p, vcnt.p, wc.p, wc_inv.p, gam.p, kc.p, rc.p);
         PA_LAUNCH_CHECK();
+        // the dynamic vertices with contacts, in ascending order, for the per-sweep gather
+        hipLaunchKernelGGL(k_contact_flags, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const uint32_t*)vptr.p, tmp.p);
+        PA_LAUNCH_CHECK();
+        scan_exclusive(tmp.p, tmp.p, n, cvcount.p, scratch, stream);
+        hipLaunchKernelGGL(k_contact_compact, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const uint32_t*)vptr.p,
+                           (const uint32_t*)tmp.p, cverts.p);
+        PA_LAUNCH_CHECK();
     }
 }
 
@@ -2007,7 +2031,9 @@ void Backend::contact_sweep_once() {
     hipLaunchKernelGGL(k_contact_du, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, (float)cfg.mu, det->c.v.p, det->c.bary.p,
                        det->c.nrm.p, kc.p, p.p, rc.p, du.p);
     PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, p.p, vptr.p, vent.p, gam.p, du.p);
+    const int64_t maxv = std::min<int64_t>(n, 4 * (int64_t)nc); // vertices with contacts, at most
+    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(maxv, BLK)), dim3(BLK), 0, stream, (const uint32_t*)cvcount.p,
+                       (const uint32_t*)cverts.p, p.p, vptr.p, vent.p, (const float4*)gam.p, (const float4*)du.p);
     PA_LAUNCH_CHECK();
 }
 
@@ -2773,7 +2799,7 @@ int Backend::admm_loop(pa_step_stats& S, float eps, float gam_, float hinv) {
                 break;
             }
             try {
-                det->detect(x.p);
+                det->detect(x.p, it > 0); // rebuild at the step's first detection, refit later ones
             } catch (...) {
                 if (overlap) join(); // the rollback writes x, which the side stream reads
                 throw;
