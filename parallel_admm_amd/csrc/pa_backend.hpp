@@ -107,7 +107,6 @@ struct Backend {
     std::unique_ptr<Detector> own_det;
     int nc = 0;
     DBuf<float4> gam, kc, rc, du;
-    DBuf<uint32_t> cverts, cvcount; // dynamic vertices with contacts (ascending) and their number
     DBuf<uint32_t> vcnt, vptr, vfill, vent;
     PrimScratch scratch;
     // warm start of the contact duals at each detection (pa_warm_start)

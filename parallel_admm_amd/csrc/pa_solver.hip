@@ -520,27 +520,12 @@ __global__ void k_contact_du(int nc, float mu, const int4* __restrict__ cv, cons
     rc[c] = make_float4(r.x + d.x, r.y + d.y, r.z + d.z, 0.f);
 }
 
-// p_v += sum Gamma du over the contacts of v, for the *count dynamic vertices
-// that have contacts (cverts, ascending).
-__global__ void k_vertex_gather(const uint32_t* __restrict__ count, const uint32_t* __restrict__ cverts, float* __restrict__ p,
-                                const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent, const float4* __restrict__ G,
-                                const float4* __restrict__ du) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int)*count) return;
-    const int i = (int)cverts[t];
+__global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent,
+                                const float4* __restrict__ G, const float4* __restrict__ du) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (vptr[i] == vptr[i + 1]) return;
     st3(p, i, ld3(p, i) + gather_contacts(i, vptr, vent, G, du));
-}
-
-// flag[i] = 1 if dynamic vertex i has contacts
-__global__ void k_contact_flags(int n, const uint32_t* __restrict__ vptr, uint32_t* __restrict__ flag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flag[i] = vptr[i + 1] > vptr[i] ? 1u : 0u;
-}
-
-// cverts[pos[i]] = i for flagged vertices: the vertices with contacts in ascending order
-__global__ void k_contact_compact(int n, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ pos, uint32_t* __restrict__ cverts) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && vptr[i + 1] > vptr[i]) cverts[pos[i]] = (uint32_t)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -1323,36 +1308,68 @@ __global__ void __launch_bounds__(PBLK) k_rz_part(int n, const float* __restrict
 }
 
 // Coarsest level: x = A_c^-1 b with the dense inverse (nc <= 2048, rows
-// padded with zeros to ncp, a multiple of 4, so every lane reads float4), one
-// wave per row; b is staged in LDS (zero beyond nc).
+// padded with zeros to ncp, a multiple of 4, so every lane reads float4).
+// b is staged in LDS as three column vectors (zero beyond nc), so a lane's float4 of
+// the inverse meets one float4 of each; every wave computes CROWS rows at once
+// (independent accumulators), which also cuts the number of workgroups that
+// stage b.
+constexpr int CROWS = 2;
+constexpr int CSLOTS = 2048 / 4 / 64; // float4 slots per lane for ncp <= 2048
 __global__ void __launch_bounds__(BLK) k_amg_coarse(int nc, int ncp, const float4* __restrict__ inv, const float* __restrict__ b,
                                                     float* __restrict__ x) {
-    __shared__ float sb[3 * 2048];
-    for (int j = threadIdx.x; j < 3 * ncp; j += BLK) sb[j] = j < 3 * nc ? b[j] : 0.f;
+    __shared__ float4 sb[3][512]; // 3 x 2048 columns
+    float* sbf = &sb[0][0].x;
+    for (int j = threadIdx.x; j < ncp; j += BLK) {
+        const bool in = j < nc;
+        sbf[j] = in ? b[3 * j] : 0.f;
+        sbf[2048 + j] = in ? b[3 * j + 1] : 0.f;
+        sbf[4096 + j] = in ? b[3 * j + 2] : 0.f;
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * (BLK / 64) + (threadIdx.x >> 6);
-    if (row >= nc) return;
-    const float4* rp = inv + (size_t)row * (ncp / 4);
-    float s0 = 0.f, s1 = 0.f, s2 = 0.f;
-#pragma unroll 4
-    for (int j4 = lane; j4 < ncp / 4; j4 += 64) {
-        const float4 a = rp[j4];
-        const float* bb = sb + 12 * j4;
-        s0 += a.x * bb[0] + a.y * bb[3] + a.z * bb[6] + a.w * bb[9];
-        s1 += a.x * bb[1] + a.y * bb[4] + a.z * bb[7] + a.w * bb[10];
-        s2 += a.x * bb[2] + a.y * bb[5] + a.z * bb[8] + a.w * bb[11];
+    const int row0 = (blockIdx.x * (BLK / 64) + (threadIdx.x >> 6)) * CROWS;
+    if (row0 >= nc) return;
+    float s[CROWS][3];
+#pragma unroll
+    for (int r = 0; r < CROWS; r++) s[r][0] = s[r][1] = s[r][2] = 0.f;
+    const int q4 = ncp / 4;
+    // all of a lane's loads first (up to CSLOTS per row in flight), then the sums
+    float4 a[CROWS][CSLOTS];
+#pragma unroll
+    for (int r = 0; r < CROWS; r++) {
+        const int row = min(row0 + r, nc - 1); // rows past nc repeat the last one and are not stored
+#pragma unroll
+        for (int k = 0; k < CSLOTS; k++) {
+            const int j4 = lane + 64 * k;
+            a[r][k] = j4 < q4 ? inv[(size_t)row * q4 + j4] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        s0 += __shfl_xor(s0, o, 64);
-        s1 += __shfl_xor(s1, o, 64);
-        s2 += __shfl_xor(s2, o, 64);
+#pragma unroll
+    for (int k = 0; k < CSLOTS; k++) {
+        const int j4 = lane + 64 * k;
+        if (j4 >= q4) break;
+        const float4 bx = sb[0][j4], by = sb[1][j4], bz = sb[2][j4];
+#pragma unroll
+        for (int r = 0; r < CROWS; r++) {
+            const float4 v = a[r][k];
+            s[r][0] += v.x * bx.x + v.y * bx.y + v.z * bx.z + v.w * bx.w;
+            s[r][1] += v.x * by.x + v.y * by.y + v.z * by.z + v.w * by.w;
+            s[r][2] += v.x * bz.x + v.y * bz.y + v.z * bz.z + v.w * bz.w;
+        }
     }
-    if (lane == 0) {
-        x[3 * row] = s0;
-        x[3 * row + 1] = s1;
-        x[3 * row + 2] = s2;
-    }
+#pragma unroll
+    for (int r = 0; r < CROWS; r++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            for (int o = 32; o > 0; o >>= 1) s[r][k] += __shfl_xor(s[r][k], o, 64);
+    if (lane == 0)
+#pragma unroll
+        for (int r = 0; r < CROWS; r++)
+            if (row0 + r < nc) {
+                x[3 * (row0 + r)] = s[r][0];
+                x[3 * (row0 + r) + 1] = s[r][1];
+                x[3 * (row0 + r) + 2] = s[r][2];
+            }
 }
 
 // Rigid animation of static vertices (reference ScriptAnimator::animate_rigid,
@@ -1914,8 +1931,6 @@ void Backend::assemble_and_upload(const double* x0h) {
     rc.alloc(cap);
     du.alloc(cap);
     vcnt.alloc(n_all);
-    cverts.alloc(std::max(n, 1));
-    cvcount.alloc(1);
     vptr.alloc((size_t)n + 1);
     vfill.alloc(n);
     vent.alloc(4 * (size_t)cap);
@@ -2008,13 +2023,6 @@ void Backend::contact_setup(float eps, float gamma_, float hinv) {
         hipLaunchKernelGGL(k_contact_scc, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, eps, gamma_, hinv, det->c.v.p,
                            det->c.bary.p, det->c.nrm.p, vcnt.p, wc.p, wc_inv.p, gam.p, kc.p, rc.p);
         PA_LAUNCH_CHECK();
-        // the dynamic vertices with contacts, in ascending order, for the per-sweep gather
-        hipLaunchKernelGGL(k_contact_flags, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const uint32_t*)vptr.p, tmp.p);
-        PA_LAUNCH_CHECK();
-        scan_exclusive(tmp.p, tmp.p, n, cvcount.p, scratch, stream);
-        hipLaunchKernelGGL(k_contact_compact, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const uint32_t*)vptr.p,
-                           (const uint32_t*)tmp.p, cverts.p);
-        PA_LAUNCH_CHECK();
     }
 }
 
@@ -2031,9 +2039,7 @@ void Backend::contact_sweep_once() {
     hipLaunchKernelGGL(k_contact_du, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, (float)cfg.mu, det->c.v.p, det->c.bary.p,
                        det->c.nrm.p, kc.p, p.p, rc.p, du.p);
     PA_LAUNCH_CHECK();
-    const int64_t maxv = std::min<int64_t>(n, 4 * (int64_t)nc); // vertices with contacts, at most
-    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(maxv, BLK)), dim3(BLK), 0, stream, (const uint32_t*)cvcount.p,
-                       (const uint32_t*)cverts.p, p.p, vptr.p, vent.p, (const float4*)gam.p, (const float4*)du.p);
+    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, p.p, vptr.p, vent.p, gam.p, du.p);
     PA_LAUNCH_CHECK();
 }
 
@@ -2131,7 +2137,7 @@ void Backend::precondition(const float* r, bool skip_pre0) {
     DevLevel& Cl = *amg[nl - 1];
     const float* bc = nl == 1 ? r : Cl.b.p;
     float* xc = nl == 1 ? cg_z.p : Cl.x2.p;
-    hipLaunchKernelGGL(k_amg_coarse, dim3(grid_for(amg_nc, BLK / 64)), dim3(BLK), 0, stream, amg_nc, amg_ncp,
+    hipLaunchKernelGGL(k_amg_coarse, dim3(grid_for(amg_nc, (BLK / 64) * CROWS)), dim3(BLK), 0, stream, amg_nc, amg_ncp,
                        (const float4*)amg_cinv.p, bc, xc);
     PA_LAUNCH_CHECK();
     if (nl == 1) { // the whole system is the coarse level: exact solve, then r.z partials
