@@ -160,3 +160,43 @@ def test_batch_of_uneven_scenes_runs(pa, precond):
         assert it.max() < cfg.admm_max_iter * cfg.global_max_iter
     x, _ = b.get_state()
     assert np.all(np.isfinite(x))
+
+
+def test_batch_with_warm_start_and_residual_stop(pa):
+    """A batch with the residual stop test on (over the whole batch) and the
+    previous-window warm start: every step ends finite, stops at the
+    tolerance or the cap, and a step judged bad re-runs cold."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.residual_tol = 1e-4
+    parts, merged, seg, xm, _ = _batch(pa, cfg, frac=2.0)
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_state(x=xm)
+    b.set_warm_start(2)
+    seeded = 0
+    for f in range(6):
+        st = b.step()
+        seeded += st["ws_seeded"]
+        assert st["converged"] == 1 or st["iterations"] == cfg.admm_max_iter
+        print(f"frame {f}: iterations {st['iterations']}, converged {st['converged']}, contacts {st['n_contacts']}, "
+              f"seeded {st['ws_seeded']}, fallback {st['ws_fallback']}")
+    x, v = b.get_state()
+    assert np.all(np.isfinite(x)) and np.all(np.isfinite(v)) and seeded > 0
+
+
+def test_batch_overflow_rolls_back(pa):
+    """Contact capacity exceeded inside a batch: PA_ERR_OVERFLOW, the true
+    count is reported and the positions are those of the step start."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("cloth32")
+    cfg.max_contacts = 8
+    parts, merged, seg, xm, _ = _batch(pa, cfg, frac=0.3)
+    b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+    b.set_state(x=xm)
+    x0, _ = b.get_state()
+    with pytest.raises(pa.PAError) as e:
+        b.step()
+    assert e.value.code == pa.PA_ERR_OVERFLOW
+    np.testing.assert_array_equal(b.get_state()[0], x0)
