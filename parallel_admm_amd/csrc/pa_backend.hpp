@@ -106,7 +106,9 @@ struct Backend {
     Detector* det = nullptr;
     std::unique_ptr<Detector> own_det;
     int nc = 0;
-    DBuf<float4> gam, kc, rc, du;
+    DBuf<float4> gam, kc, rc;
+    DBuf<int32_t> cslot;  // per contact: entry slot of each of its 4 vertices (-1: static)
+    DBuf<float4> gdu;     // per vertex entry: Gamma_k du of the last sweep
     DBuf<uint32_t> vcnt, vptr, vfill, vent;
     PrimScratch scratch;
     // warm start of the contact duals at each detection (pa_warm_start)

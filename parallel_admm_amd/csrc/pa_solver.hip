@@ -488,7 +488,8 @@ __global__ void k_vertex_p(int n, float hinv, int have_contacts, const float* __
 // (formula of ADMMImplCPU::_project_one_contact, admm_impl_cpu.cpp:411-448).
 __global__ void k_contact_du(int nc, float mu, const int4* __restrict__ cv, const float4* __restrict__ bary,
                              const float4* __restrict__ nrm, const float4* __restrict__ K, const float* __restrict__ p,
-                             float4* __restrict__ rc, float4* __restrict__ du) {
+                             float4* __restrict__ rc, const float4* __restrict__ G, const int4* __restrict__ slot,
+                             float4* __restrict__ gdu) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= nc) return;
     int4 v = cv[c];
@@ -516,16 +517,37 @@ __global__ void k_contact_du(int nc, float mu, const int4* __restrict__ cv, cons
     }
     float3 unew = uN * nn + uT;
     float3 d = unew - u;
-    du[c] = make_float4(d.x, d.y, d.z, 0.f);
     rc[c] = make_float4(r.x + d.x, r.y + d.y, r.z + d.z, 0.f);
+    // Gamma_k du goes straight into the entry slot of vertex k (dynamic
+    // vertices only), so the per-vertex gather reads contiguous entries
+    const float4 g = G[c];
+    const int4 sl = slot[c];
+    const float gk[4] = {g.x, g.y, g.z, g.w};
+    const int sk[4] = {sl.x, sl.y, sl.z, sl.w};
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+        if (sk[k] >= 0) gdu[sk[k]] = make_float4(gk[k] * d.x, gk[k] * d.y, gk[k] * d.z, 0.f);
 }
 
-__global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent,
-                                const float4* __restrict__ G, const float4* __restrict__ du) {
+// slot[c].k = position of (contact c, vertex k) in the sorted vertex entry list, -1 for static vertices
+__global__ void k_contact_slots(int n, const uint32_t* __restrict__ vptr, const uint32_t* __restrict__ vent, int32_t* __restrict__ slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t e = vptr[i]; e < vptr[i + 1]; e++) slot[vent[e]] = (int32_t)e;
+}
+
+// p_v += sum of the Gamma du entries of v (contact order)
+__global__ void k_vertex_gather(int n, float* __restrict__ p, const uint32_t* __restrict__ vptr, const float4* __restrict__ gdu) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (vptr[i] == vptr[i + 1]) return;
-    st3(p, i, ld3(p, i) + gather_contacts(i, vptr, vent, G, du));
+    const uint32_t e0 = vptr[i], e1 = vptr[i + 1];
+    if (e0 == e1) return;
+    float3 s = f3(0.f, 0.f, 0.f);
+    for (uint32_t e = e0; e < e1; e++) {
+        const float4 g = gdu[e];
+        s = s + f3(g.x, g.y, g.z);
+    }
+    st3(p, i, ld3(p, i) + s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1929,7 +1951,8 @@ void Backend::assemble_and_upload(const double* x0h) {
     gam.alloc(cap);
     kc.alloc(cap);
     rc.alloc(cap);
-    du.alloc(cap);
+    cslot.alloc(4 * (size_t)cap);
+    gdu.alloc(4 * (size_t)cap);
     vcnt.alloc(n_all);
     vptr.alloc((size_t)n + 1);
     vfill.alloc(n);
@@ -2020,6 +2043,10 @@ void Backend::contact_setup(float eps, float gamma_, float hinv) {
         PA_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_contact_sort, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, vptr.p, vent.p);
         PA_LAUNCH_CHECK();
+        fill<int32_t>(cslot.p, -1, 4 * (int64_t)nc, stream);
+        hipLaunchKernelGGL(k_contact_slots, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, (const uint32_t*)vptr.p, (const uint32_t*)vent.p,
+                           cslot.p);
+        PA_LAUNCH_CHECK();
         hipLaunchKernelGGL(k_contact_scc, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, n, eps, gamma_, hinv, det->c.v.p,
                            det->c.bary.p, det->c.nrm.p, vcnt.p, wc.p, wc_inv.p, gam.p, kc.p, rc.p);
         PA_LAUNCH_CHECK();
@@ -2037,9 +2064,9 @@ void Backend::contact_project(bool) {
 
 void Backend::contact_sweep_once() {
     hipLaunchKernelGGL(k_contact_du, dim3(grid_for(nc, BLK)), dim3(BLK), 0, stream, nc, (float)cfg.mu, det->c.v.p, det->c.bary.p,
-                       det->c.nrm.p, kc.p, p.p, rc.p, du.p);
+                       det->c.nrm.p, kc.p, p.p, rc.p, (const float4*)gam.p, (const int4*)cslot.p, gdu.p);
     PA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, p.p, vptr.p, vent.p, gam.p, du.p);
+    hipLaunchKernelGGL(k_vertex_gather, dim3(grid_for(n, BLK)), dim3(BLK), 0, stream, n, p.p, vptr.p, (const float4*)gdu.p);
     PA_LAUNCH_CHECK();
 }
 
