@@ -69,7 +69,7 @@ typedef struct pa_admm_config {
     double residual_tol;    /* stop when max(r_primal, r_dual) <= tol; 0 = run admm_max_iter */
     int32_t residual_interval; /* check the residual every k iterations */
     int32_t global_max_iter;   /* cap on PCG iterations per refinement pass (Global_Jacobi_iter) */
-    double global_tol;         /* PCG stop per pass and column: |r_k| <= global_tol |r_0|, r_0 = b - A x in FP64 */
+    double global_tol;         /* PCG stop per pass and column: |r_k| <= global_tol |r_0|, r_0 = b - A x in FP64; default 2e-3 */
     int32_t max_contacts;      /* narrow.max_collision: contact capacity */
     int32_t global_refine;     /* refinement passes per global solve (FP64 residual, FP32 PCG correction) */
     int32_t precond;           /* PCG preconditioner: 0 Jacobi, 1 smoothed-aggregation AMG V-cycle */

@@ -167,7 +167,7 @@ class AdmmConfig:
     residual_tol: float = 0.0
     residual_interval: int = 1
     global_max_iter: int = 200
-    global_tol: float = 1e-3
+    global_tol: float = 2e-3
     max_contacts: int = 1 << 20
     global_refine: int = 1
     precond: int = 1

@@ -84,7 +84,7 @@ void pa_admm_config_default(pa_admm_config* c) {
     c->residual_tol = 0.0;
     c->residual_interval = 1;
     c->global_max_iter = 200;
-    c->global_tol = 1e-3;
+    c->global_tol = 2e-3;
     c->max_contacts = 1 << 20;
     c->global_refine = 1;
     c->precond = 1;
