@@ -64,16 +64,24 @@ def copy_calibration() -> int:
     return 0
 
 
-def cpu_baseline(workload: str, scene, cfg, state, tracks, frame: int, seconds: float) -> dict:
-    """FP64 oracle (prefactored sparse LLT, OpenMP local step and broad phase)
-    on the host cores, started from the GPU's state at the beginning of its
-    timed region (obstacle tracks resumed at the same frame) and run for about
-    `seconds` (at least one step)."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _oracle_run(prec: int, scene, cfg, state, tracks, frame: int, seconds: float) -> dict:
+    """Steps the oracle (prec 64: FP64; prec 32: the FP32 twin) from `state`
+    for about `seconds` (at least one step); steps/s, ms per step and the
+    phase split (detection, local, contact, global)."""
     from oracle.pyoracle import Oracle
 
-    cores = os.cpu_count() or 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
-    o = Oracle(scene.mesh(), cfg)
+    o = Oracle(scene.mesh(), cfg, prec=prec)
     for first, count, pos, quat in tracks:
         o.add_rigid_track(first, count, pos[frame:], quat[frame:])
     x, v, ue, uc = state
@@ -81,16 +89,34 @@ def cpu_baseline(workload: str, scene, cfg, state, tracks, frame: int, seconds: 
     o.set_duals(ue, uc)
     t0 = time.perf_counter()
     steps, iters = 0, 0
+    ph = np.zeros(4)
     while True:
         st = o.step()
         steps += 1
         iters += st["iterations"]
+        ph += [st["dcd_ms"], st["local_ms"], st["contact_ms"], st["global_ms"]]
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": steps / dt, "unit": "steps/s", "cores": threads, "kind": "port",
-            "sample": f"{workload}: {steps} FP64 oracle steps ({iters} ADMM iterations) from the GPU's state at the "
-                      f"start of its timed region, {dt:.1f} s"}
+    return {"value": steps / dt, "ms_per_step": 1e3 * dt / steps, "steps": steps, "admm_iterations": iters, "seconds": dt,
+            "phase_ms": dict(zip(("detection", "local", "contact", "global"), [round(float(t), 2) for t in ph / steps]))}
+
+
+def cpu_baseline(workload: str, scene, cfg, state, tracks, frame: int, seconds: float) -> dict:
+    """The CPU restatement of the reference's CPU backend (prefactored sparse
+    LLT, OpenMP local step and broad phase) on the host cores, started from
+    the GPU's state at the beginning of its timed region (obstacle tracks
+    resumed at the same frame): the FP64 oracle for about `seconds`, then its
+    FP32 twin (the reference's own precision) for about half that."""
+    cores = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", cores))
+    f64 = _oracle_run(64, scene, cfg, state, tracks, frame, seconds)
+    f32 = _oracle_run(32, scene, cfg, state, tracks, frame, seconds / 2)
+    return {"value": f64["value"], "unit": "steps/s", "cores": threads, "kind": "port",
+            "sample": f"{workload}: {f64['steps']} FP64 oracle steps ({f64['admm_iterations']} ADMM iterations) from the GPU's "
+                      f"state at the start of its timed region, {f64['seconds']:.1f} s",
+            "cpu_model": _cpu_model(), "host_cpus": cores, "ms_per_step": f64["ms_per_step"], "phase_ms": f64["phase_ms"],
+            "fp32_twin": {k: f32[k] for k in ("value", "ms_per_step", "steps", "phase_ms")}}
 
 
 MLP_FLOPS_FWD = 2 * (10 * 128 + 2 * 128 * 128 + 128 * 2)  # per sample, algorithmic (unpadded)
@@ -204,6 +230,7 @@ def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) ->
         "solo_scene_steps_per_s": float(1000.0 / solo_ms.mean()),
         "batch_speedup_vs_one_at_a_time": float((count * steps / wall) / (1000.0 / solo_ms.mean())),
         "global_solve": "AMG-PCG over the block-diagonal batch, stopped per scene",
+        "pcg_iteration_pairs_as_hip_graphs": bool(cfg.graphs),
     }
     print(f"[bench] batch of {count}: {out['batched_steps_per_s']:.2f} batched steps/s = {out['scene_steps_per_s']:.0f} scene-steps/s; "
           f"one at a time {out['solo_scene_steps_per_s']:.0f} scene-steps/s (step ms p50 {out['solo_step_ms_p50']:.1f}, "

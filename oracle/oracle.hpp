@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1221,6 +1222,10 @@ struct StepStats {
     int n_contacts = 0;
     double r_primal = 0, r_dual = 0;
     int converged = 0;
+    // wall-clock ms per phase, named as the reference's Timer tags
+    // (admm_impl_cpu.cpp step_fast): collision detection, local (elastic
+    // prox), contact projection, global (RHS and the three LLT solves)
+    double t_dcd = 0, t_local = 0, t_contact = 0, t_global = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1570,7 +1575,16 @@ template <class R> struct Solver {
         std::vector<R> z(3 * m, R(0)), zprev(3 * m, R(0)), DX, t1(3 * n), b(3 * n), tmp;
         bool have_contacts = false;
         int it = 0, ndet = 0;
+        using clk = std::chrono::steady_clock;
+        auto lap = [](clk::time_point& t) {
+            const clk::time_point now = clk::now();
+            const double ms = std::chrono::duration<double, std::milli>(now - t).count();
+            t = now;
+            return ms;
+        };
+        clk::time_point tp = clk::now();
         for (; it < cfg.admm_max_iter; it++) {
+            (void)lap(tp); // the residual check of the previous iteration is in no phase
             if (cfg.enable_contact && (it % cfg.dcd_interval == 0)) {
                 contacts = detect<R>(T, xc.data(), R(cfg.thickness), brute_force, seg.empty() ? nullptr : &seg);
                 compute_scc(eps, gam, hinv);
@@ -1579,12 +1593,15 @@ template <class R> struct Solver {
                 ndet++;
                 have_contacts = true;
             }
+            st.t_dcd += lap(tp);
             zprev = z;
             project_elastic(xc, z);
+            st.t_local += lap(tp);
             if (cfg.enable_contact) {
                 for (int i = 0; i < 3 * n; i++) p[i] = (xc[i] - x0[i] + Uc[i]) * hinv;
                 if (have_contacts) project_contacts(p);
             }
+            st.t_contact += lap(tp);
             // b = M x~ + h^2 D^T We^2 (z - Ue) + h^2 Wc (h p + x0 - Uc) + b_ini
             tmp.resize(3 * m);
             for (int i = 0; i < 3 * m; i++) tmp[i] = z[i] - Ue[i];
@@ -1610,10 +1627,12 @@ template <class R> struct Solver {
                 }
                 for (int i = 0; i < 3 * n; i++) xc[i] = R(xd[i]);
             }
+            st.t_global += lap(tp);
             apply_D(xc, DX);
             for (int i = 0; i < 3 * m; i++) Ue[i] += DX[i] - z[i];
             if (cfg.enable_contact)
                 for (int i = 0; i < 3 * n; i++) Uc[i] += xc[i] - x0[i] - h * p[i];
+            st.t_global += lap(tp);
             if (cfg.residual_tol > 0 && ((it + 1) % cfg.residual_interval == 0)) {
                 double rp, rd;
                 residual(xc, x0, p, z, zprev, DX, &rp, &rd);

@@ -57,6 +57,7 @@ struct orc_config {
 struct orc_stats {
     int iterations, n_contacts, converged, pad;
     double r_primal, r_dual, step_ms;
+    double dcd_ms, local_ms, contact_ms, global_ms;
 };
 
 void* orc_create(int prec, int n_all, int n_dyn, const double* x, const double* density, int n_tri, const int* tris,
@@ -131,6 +132,10 @@ int orc_step(void* hp, orc_stats* out) {
         out->r_primal = st.r_primal;
         out->r_dual = st.r_dual;
         out->step_ms = ms;
+        out->dcd_ms = st.t_dcd;
+        out->local_ms = st.t_local;
+        out->contact_ms = st.t_contact;
+        out->global_ms = st.t_global;
     }
     return 0;
 }

@@ -23,7 +23,8 @@ class OrcConfig(C.Structure):
 
 class OrcStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("n_contacts", C.c_int32), ("converged", C.c_int32), ("pad", C.c_int32),
-                ("r_primal", C.c_double), ("r_dual", C.c_double), ("step_ms", C.c_double)]
+                ("r_primal", C.c_double), ("r_dual", C.c_double), ("step_ms", C.c_double),
+                ("dcd_ms", C.c_double), ("local_ms", C.c_double), ("contact_ms", C.c_double), ("global_ms", C.c_double)]
 
 
 _lib = None

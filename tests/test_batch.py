@@ -110,6 +110,30 @@ def test_batched_step_matches_oracle_and_solo(pa, precond):
     assert worst_o < 1e-5 and worst_s < 1e-5 and contacts > 0
 
 
+def test_batched_graph_replay_matches_eager(pa):
+    """The batched whole-system AMG-PCG with its iteration pairs replayed as
+    HIP graphs (per-scene stop test at every read-back between replays)
+    against every kernel launched one by one: bitwise identical states and
+    iteration counts."""
+    from parallel_admm_amd.configs import config_for
+
+    out = {}
+    for graphs in (0, 1):
+        cfg = config_for("cloth32")
+        cfg.graphs = graphs
+        parts, merged, seg, xm, _ = _batch(pa, cfg, frac=2.0)
+        b = pa.ADMMBackendHIP.from_scene(merged, cfg)
+        b.set_state(x=xm)
+        out[graphs] = []
+        for _ in range(3):
+            st = b.step()
+            out[graphs].append((b.get_state()[0], st["global_iters"], b.batch_stats(len(parts))[0]))
+    for (xe, ie, se), (xg, ig, sg) in zip(out[0], out[1]):
+        assert ie == ig and ie > 0
+        np.testing.assert_array_equal(sg, se)
+        np.testing.assert_array_equal(xg, xe)
+
+
 def test_batched_solve_variants_agree_and_autotune(pa):
     from parallel_admm_amd.configs import config_for
 
