@@ -769,7 +769,7 @@ __global__ void __launch_bounds__(BLK) k_resid_final(const double* __restrict__ 
 // src/solver/jacobi_solver.cu:141-180).  Three right-hand sides (xyz) share
 // each SpMV; FP32 vectors, FP64 dot products.  cg_state layout (two slots s,
 // ping-pong by iteration parity): [s*8 + 0..2] rz, [s*8 + 3..5] done flags;
-// [16..18] bb; [19] iterations.
+// [16..18] stop thresholds tol^2 |r_0|^2; [19] iterations; [20..22] |b|^2.
 // ---------------------------------------------------------------------------
 // Row groups: LPR consecutive lanes share one CSR row, each taking every
 // LPR-th entry, and combine with shuffles (width LPR).  Many short dependent
