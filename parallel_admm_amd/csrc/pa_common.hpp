@@ -111,3 +111,16 @@ __device__ inline void st3(float* p, int64_t i, float3 v) {
     p[3 * i + 1] = v.y;
     p[3 * i + 2] = v.z;
 }
+
+// XCD-contiguous block numbering.  The dispatcher deals workgroups to the
+// eight XCDs round-robin (workgroup b runs on XCD b % 8) and each XCD has its
+// own L2, so with blockIdx.x as the work index every XCD would touch every
+// part of the mesh.  This bijection of [0, gridDim.x) hands XCD x one
+// contiguous range of logical blocks instead (XCD x holds q + (x < r) of the
+// grid's workgroups, q = grid / 8, r = grid % 8): neighbouring rows and
+// queries, whose gathers share vertices and tree nodes, then share an L2.
+__device__ inline int xcd_block() {
+    const int g = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = g >> 3, r = g & 7, x = b & 7, k = b >> 3;
+    return x * q + min(x, r) + k;
+}

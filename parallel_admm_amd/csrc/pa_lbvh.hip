@@ -690,24 +690,26 @@ int Detector::detect(const float* x, bool refit) {
     if (ne > 0) fill<uint32_t>(qcount.p + nv, 0u, ne, stream);
     PA_HIP(hipEventRecord(ev_fork, stream));
     PA_HIP(hipStreamWaitEvent(side, ev_fork, 0));
-    build_tree(ft, x, true, stream, scratch, refit);
-    build_tree(et, x, false, side, scratch_side, refit);
-    built = true;
     const float gap = (float)(dhat * 1.001);
     QueryOut o{c.type.p, c.prim0.p, c.prim1.p, c.v.p, c.bary.p, c.nrm.p, c.dist.p, c.cap};
     QueryOut st{stg.type.p, stg.prim0.p, stg.prim1.p, stg.v.p, stg.bary.p, stg.nrm.p, stg.dist.p, stg.cap};
-    // one traversal per query: pairs to the staging list (total[2] counts
-    // them, atomically from both streams), per-segment counts to qcount
-    if (nv > 0) {
-        hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, x, surf_verts.p, nv, tris.p, n_dyn, nf, ft.rec.p,
-                           ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
-                           nseg > 0 ? (const int32_t*)ft.seg_root.p : nullptr);
-        PA_LAUNCH_CHECK();
-    }
+    // One traversal per query: pairs to the staging list (total[2] counts
+    // them, atomically from both streams), per-segment counts to qcount.  The
+    // edge tree and its query, the longer chain, are enqueued first so the
+    // side stream starts while the host still enqueues the triangle tree.
+    build_tree(et, x, false, side, scratch_side, refit);
     if (ne > 0) {
         hipLaunchKernelGGL(k_query_ee, dim3(grid_for(ne, QBLK)), dim3(QBLK), 0, side, x, edges.p, ne, n_dyn, et.rec.p, et.ids.p, dhat,
                            gap, qcount.p + nv, nv, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
                            nseg > 0 ? (const int32_t*)et.seg_root.p : nullptr);
+        PA_LAUNCH_CHECK();
+    }
+    build_tree(ft, x, true, stream, scratch, refit);
+    built = true;
+    if (nv > 0) {
+        hipLaunchKernelGGL(k_query_pt, dim3(grid_for(nv, QBLK)), dim3(QBLK), 0, stream, x, surf_verts.p, nv, tris.p, n_dyn, nf, ft.rec.p,
+                           ft.ids.p, dhat, gap, qcount.p, st, st_seg.p, total.p + 2, total.p, (const int32_t*)vseg.p,
+                           nseg > 0 ? (const int32_t*)ft.seg_root.p : nullptr);
         PA_LAUNCH_CHECK();
     }
     PA_HIP(hipEventRecord(ev_join, side));

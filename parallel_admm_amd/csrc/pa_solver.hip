@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(PBLK) k_cg_resid64(int n, const int32_t* __res
                                                     float* __restrict__ d, double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
     double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
+    for (int r0 = xcd_block() * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
         const int i = r0 + threadIdx.x / LPR;
         double ax[3] = {0, 0, 0};
         if (i < n) {
@@ -929,7 +929,7 @@ __global__ void __launch_bounds__(PBLK) k_cg_spmv(int n, const int32_t* __restri
                                                  const float* __restrict__ pd, float* __restrict__ q, double* __restrict__ part) {
     const int sub = threadIdx.x % LPR;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
+    for (int r0 = xcd_block() * (PBLK / LPR); r0 < n; r0 += gridDim.x * (PBLK / LPR)) {
         const int i = r0 + threadIdx.x / LPR;
         float3 s = row_dot<LPR>(i, n, sub, ptr, col, val, pd);
         if (i < n && sub == 0) {
@@ -1079,7 +1079,7 @@ __global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot
     bool need_be = !first;
     double acc[3] = {0, 0, 0};
     auto comb = [&](float3 zc, float3 pc) { return f3(zc.x + be[0] * pc.x, zc.y + be[1] * pc.y, zc.z + be[2] * pc.z); };
-    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
@@ -1167,7 +1167,7 @@ __global__ void __launch_bounds__(PBLK) k_cg_update_pre(int n, int slot, int npa
     };
     auto rnew = [&](float3 rc, float3 qc) { return f3(rc.x - al[0] * qc.x, rc.y - al[1] * qc.y, rc.z - al[2] * qc.z); };
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
@@ -1243,7 +1243,7 @@ __global__ void __launch_bounds__(BLK) k_amg_pre(int n, const int32_t* __restric
                                                  const float* __restrict__ val, const float* __restrict__ dinv, float w,
                                                  const float* __restrict__ b, float* __restrict__ x, float* __restrict__ r) {
     const int sub = threadIdx.x % LPR;
-    const int i = blockIdx.x * (BLK / LPR) + threadIdx.x / LPR;
+    const int i = xcd_block() * (BLK / LPR) + threadIdx.x / LPR;
     float3 s = f3(0.f, 0.f, 0.f);
     row_foreach<LPR>(i, n, sub, ptr, col, val, [&](float v, int c) { s = s + (v * (w * dinv[c])) * ld3(b, c); });
     s = group_sum<LPR>(s);
@@ -1260,7 +1260,7 @@ __global__ void __launch_bounds__(BLK) k_amg_spmv(int rows, const int32_t* __res
                                                   const float* __restrict__ val, const float* __restrict__ in,
                                                   const float* __restrict__ base, float* __restrict__ out) {
     const int sub = threadIdx.x % LPR;
-    const int i = blockIdx.x * (BLK / LPR) + threadIdx.x / LPR;
+    const int i = xcd_block() * (BLK / LPR) + threadIdx.x / LPR;
     float3 s = row_dot<LPR>(i, rows, sub, ptr, col, val, in);
     if (i < rows && sub == 0) st3(out, i, base ? ld3(base, i) + s : s);
 }
@@ -1276,7 +1276,7 @@ __global__ void __launch_bounds__(PBLK) k_amg_post(int n, const int32_t* __restr
     const int sub = threadIdx.x % LPR;
     const int rpb = PBLK / LPR, stride = gridDim.x * rpb;
     double acc[3] = {0, 0, 0};
-    for (int r0 = blockIdx.x * rpb; r0 < n; r0 += 2 * stride) {
+    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
