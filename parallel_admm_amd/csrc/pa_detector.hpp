@@ -16,7 +16,7 @@ struct Tree {
     DBuf<uint32_t> keys, ids; // sorted Morton codes and primitive ids
     DBuf<int32_t> child;     // 2 per internal node
     DBuf<int2> range;        // per internal node: first and last sorted leaf it covers
-    DBuf<float4> rec;        // 4 per internal node: child boxes, child ids, last leaf under each child
+    DBuf<float4> rec;        // 8 per internal node: boxes, ids and last leaf of its up to four grandchildren
     DBuf<float4> chunk64;    // 2 per 64-leaf chunk: union of its leaf boxes
     DBuf<float4> chunk4k;    // 2 per 4096-leaf chunk
     DBuf<float4> scene;      // 2: scene box

@@ -3,8 +3,9 @@
 // Per detection: primitive boxes -> scene box -> 30-bit Morton codes ->
 // stable LDS radix sort -> Karras radix tree (T. Karras, "Maximizing
 // Parallelism in the Construction of BVHs, Octrees, and k-d Trees", HPG 2012)
-// -> node boxes from leaf ranges -> 64-byte node records (both child boxes,
-// child ids and the last sorted leaf under each child, so a traversal step is
+// -> node boxes from leaf ranges -> 128-byte wide node records (the boxes,
+// ids and last sorted leaf of a node's up to four grandchildren, a leaf child
+// standing in for its own pair, so a traversal step descends two levels with
 // one dependent load) -> one traversal per query with the FP64 predicate,
 // accepted pairs appended to a staging list with their segment.
 //
@@ -263,22 +264,39 @@ __global__ void k_node_boxes(int n, const int2* __restrict__ range, const float4
     node_box[2 * i + 1] = hi;
 }
 
-// Node record i (4 float4): (left lo, left id), (left hi, right id),
-// (right lo, last leaf under left), (right hi, last leaf under right); ids and
-// leaf positions are stored as int bits in w.
+// Wide node record i (8 float4): four slots, slot j = (lo, node id) at
+// 2j and (hi, last sorted leaf under the node) at 2j + 1, ids and leaf
+// positions as int bits in w.  Slots 0-1 come from the left child, 2-3 from
+// the right: an internal child gives its two children, a leaf child itself
+// (slot 0 or 2) and an unused slot (id -1, empty box).
+__device__ inline void rec_slot(int n, int c, const int2* __restrict__ range, const float4* __restrict__ node_box, float4* out) {
+    if (c < 0) {
+        out[0] = make_float4(3e38f, 3e38f, 3e38f, __int_as_float(-1));
+        out[1] = make_float4(-3e38f, -3e38f, -3e38f, __int_as_float(-1));
+        return;
+    }
+    const int last = c >= n - 1 ? c - (n - 1) : range[c].y;
+    const float4 lo = node_box[2 * c], hi = node_box[2 * c + 1];
+    out[0] = make_float4(lo.x, lo.y, lo.z, __int_as_float(c));
+    out[1] = make_float4(hi.x, hi.y, hi.z, __int_as_float(last));
+}
+
 __global__ void k_node_records(int n, const int32_t* __restrict__ child, const int2* __restrict__ range,
                                const float4* __restrict__ node_box, float4* __restrict__ rec) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     const int l = child[2 * i], r = child[2 * i + 1];
-    const int last_l = l >= n - 1 ? l - (n - 1) : range[l].y;
-    const int last_r = r >= n - 1 ? r - (n - 1) : range[r].y;
-    const float4 llo = node_box[2 * l], lhi = node_box[2 * l + 1], rlo = node_box[2 * r], rhi = node_box[2 * r + 1];
-    rec[4 * i] = make_float4(llo.x, llo.y, llo.z, __int_as_float(l));
-    rec[4 * i + 1] = make_float4(lhi.x, lhi.y, lhi.z, __int_as_float(r));
-    rec[4 * i + 2] = make_float4(rlo.x, rlo.y, rlo.z, __int_as_float(last_l));
-    rec[4 * i + 3] = make_float4(rhi.x, rhi.y, rhi.z, __int_as_float(last_r));
+    const bool li = l < n - 1, ri = r < n - 1;
+    float4* o = rec + 8 * (size_t)i;
+    rec_slot(n, li ? child[2 * l] : l, range, node_box, o);
+    rec_slot(n, li ? child[2 * l + 1] : -1, range, node_box, o + 2);
+    rec_slot(n, ri ? child[2 * r] : r, range, node_box, o + 4);
+    rec_slot(n, ri ? child[2 * r + 1] : -1, range, node_box, o + 6);
 }
+
+// Slot j's id from the four slot ids (j is wave-divergent; selects, no
+// dynamically indexed array).
+__device__ inline int pick4(int j, int c0, int c1, int c2, int c3) { return j == 0 ? c0 : j == 1 ? c1 : j == 2 ? c2 : c3; }
 
 __device__ inline bool box_far(float4 nlo, float4 nhi, float4 qlo, float4 qhi, float gap) {
     return (nlo.x - qhi.x > gap) || (qlo.x - nhi.x > gap) || (nlo.y - qhi.y > gap) || (qlo.y - nhi.y > gap) ||
@@ -396,23 +414,34 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, 
     } else {
         int sp = 0, node = start;
         while (true) {
-            const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
-            const int l = __float_as_int(a.w), r = __float_as_int(b.w);
-            bool hl = !box_far(a, b, pf, pf, gap), hr = !box_far(c, d, pf, pf, gap);
-            // leaf children are tested here (one call site keeps the FP64 test inlined once)
-            const bool ll = hl && l >= leaf0, lr = hr && r >= leaf0;
-            for (int t = 0; t < 2; t++)
-                if (t == 0 ? ll : lr) leaf(t == 0 ? l : r);
-            hl = hl && !ll;
-            hr = hr && !lr;
-            if (hl && hr) {
-                if (sp < STACK) stack[sp++][threadIdx.x] = r;
-                else atomicOr(&flags[1], 1u);
-                node = l;
-            } else if (hl) {
-                node = l;
-            } else if (hr) {
-                node = r;
+            const float4* rn = rec + 8 * (size_t)node;
+            float4 s[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) s[j] = rn[j];
+            int c[4];
+            unsigned leaves = 0, inner = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                c[j] = __float_as_int(s[2 * j].w);
+                const bool hit = c[j] >= 0 && !box_far(s[2 * j], s[2 * j + 1], pf, pf, gap);
+                if (hit) (c[j] >= leaf0 ? leaves : inner) |= 1u << j;
+            }
+            // leaf slots are tested here (one call site keeps the FP64 test inlined once)
+            while (leaves) {
+                const int j = __ffs(leaves) - 1;
+                leaves &= leaves - 1;
+                leaf(pick4(j, c[0], c[1], c[2], c[3]));
+            }
+            if (inner) {
+                const int j = __ffs(inner) - 1;
+                inner &= inner - 1;
+                node = pick4(j, c[0], c[1], c[2], c[3]);
+                while (inner) {
+                    const int t = __ffs(inner) - 1;
+                    inner &= inner - 1;
+                    if (sp < STACK) stack[sp++][threadIdx.x] = pick4(t, c[0], c[1], c[2], c[3]);
+                    else atomicOr(&flags[1], 1u);
+                }
             } else {
                 if (sp == 0) break;
                 node = stack[--sp][threadIdx.x];
@@ -466,23 +495,34 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, 
     if (start < 0 || start >= leaf0) return;
     int sp = 0, node = start;
     while (true) {
-        const float4 a = rec[4 * node], b = rec[4 * node + 1], c = rec[4 * node + 2], d = rec[4 * node + 3];
-        const int l = __float_as_int(a.w), r = __float_as_int(b.w);
-        bool hl = __float_as_int(c.w) > k && !box_far(a, b, qlo, qhi, gap);
-        bool hr = __float_as_int(d.w) > k && !box_far(c, d, qlo, qhi, gap);
-        const bool ll = hl && l >= leaf0, lr = hr && r >= leaf0;
-        for (int t = 0; t < 2; t++)
-            if (t == 0 ? ll : lr) leaf(t == 0 ? l : r);
-        hl = hl && !ll;
-        hr = hr && !lr;
-        if (hl && hr) {
-            if (sp < STACK) stack[sp++][threadIdx.x] = r;
-            else atomicOr(&flags[1], 1u);
-            node = l;
-        } else if (hl) {
-            node = l;
-        } else if (hr) {
-            node = r;
+        const float4* rn = rec + 8 * (size_t)node;
+        float4 s[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) s[j] = rn[j];
+        int c[4];
+        unsigned leaves = 0, inner = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            c[j] = __float_as_int(s[2 * j].w);
+            // an unused slot has id -1 and last leaf -1 <= k
+            const bool hit = __float_as_int(s[2 * j + 1].w) > k && !box_far(s[2 * j], s[2 * j + 1], qlo, qhi, gap);
+            if (hit) (c[j] >= leaf0 ? leaves : inner) |= 1u << j;
+        }
+        while (leaves) {
+            const int j = __ffs(leaves) - 1;
+            leaves &= leaves - 1;
+            leaf(pick4(j, c[0], c[1], c[2], c[3]));
+        }
+        if (inner) {
+            const int j = __ffs(inner) - 1;
+            inner &= inner - 1;
+            node = pick4(j, c[0], c[1], c[2], c[3]);
+            while (inner) {
+                const int t = __ffs(inner) - 1;
+                inner &= inner - 1;
+                if (sp < STACK) stack[sp++][threadIdx.x] = pick4(t, c[0], c[1], c[2], c[3]);
+                else atomicOr(&flags[1], 1u);
+            }
         } else {
             if (sp == 0) break;
             node = stack[--sp][threadIdx.x];
@@ -551,7 +591,7 @@ Detector::Detector(int n_all_, int n_dyn_, int nv_, const int32_t* sv, int nf_, 
         T.ids.alloc(n);
         T.child.alloc(2 * (size_t)std::max(n - 1, 1));
         T.range.alloc(std::max(n - 1, 1));
-        T.rec.alloc(4 * (size_t)std::max(n - 1, 1));
+        T.rec.alloc(8 * (size_t)std::max(n - 1, 1));
         T.chunk64.alloc(2 * (size_t)((n + 63) / 64));
         T.chunk4k.alloc(2 * (size_t)((n + 4095) / 4096));
         T.scene.alloc(2);
@@ -644,7 +684,7 @@ void Detector::build_tree(Tree& T, const float* x, bool faces, hipStream_t strea
 }
 
 // Internal node boxes from the leaf boxes (chunk unions, range unions), then
-// the 64-byte node records.
+// the 128-byte wide node records.
 void Detector::node_boxes(Tree& T, hipStream_t stream) {
     const int n = T.n;
     const float4* leaves = T.node_box.p + 2 * (size_t)(n - 1);
