@@ -169,7 +169,7 @@ def learned_leg(pa, args, scene, cfg, tracks, plain_stats, plain_value) -> dict:
     }
 
 
-def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) -> dict:
+def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16, graphs: int = 1) -> dict:
     """The batched workload: `count` cloths (side ~ U{16..96}, drop ~ U(0.2,
     0.6) m, sphere radius ~ U(0.1, 0.3) m, mu ~ U(0.1, 0.6); generator seed 0;
     h = 1/60, 30 ADMM iterations) merged into one scene and stepped together.
@@ -184,6 +184,7 @@ def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16) ->
     merged = pa.Scene.merge([sc for _, sc in members])
     cfg = config_for("batch")
     cfg.max_contacts = 1 << 23
+    cfg.graphs = graphs
     b = pa.ADMMBackendHIP.from_scene(merged, cfg)
     b.set_sync_to_host(False)
     setup_s = time.perf_counter() - t0
@@ -254,6 +255,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-solo", type=int, default=16, help="scenes of the batch also stepped one at a time")
+    ap.add_argument("--batch-graphs", type=int, default=1, choices=[0, 1],
+                    help="replay the batched PCG iteration pairs as HIP graphs (0: launch every kernel, e.g. under a kernel tracer)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -266,7 +269,7 @@ def main() -> int:
     from parallel_admm_amd.configs import SCENE_KIND, config_for, workload_tracks
 
     if args.workload == "batch512":
-        bl = batch_leg(pa, args.batch_scenes or 512, args.batch_warmup, args.batch_steps, args.batch_solo)
+        bl = batch_leg(pa, args.batch_scenes or 512, args.batch_warmup, args.batch_steps, args.batch_solo, args.batch_graphs)
         print(json.dumps({
             "metric": "scene steps/s of the batched workload (512 uneven cloths stepped together)",
             "value": bl["scene_steps_per_s"], "unit": "scene-steps/s", "n_gpus": 1, "steps": args.batch_steps,
@@ -332,7 +335,7 @@ def main() -> int:
     batch = None
     if args.batch_scenes > 0:
         try:
-            batch = batch_leg(pa, args.batch_scenes, args.batch_warmup, args.batch_steps, args.batch_solo)
+            batch = batch_leg(pa, args.batch_scenes, args.batch_warmup, args.batch_steps, args.batch_solo, args.batch_graphs)
         except Exception as e:  # reported, not fatal to the headline line
             batch = {"error": f"{type(e).__name__}: {e}"}
             print(f"[bench] batched leg failed: {e}", file=sys.stderr, flush=True)
