@@ -1037,17 +1037,22 @@ __global__ void __launch_bounds__(PBLK) k_cg_state(int slot_old, int nparts, dou
 // Up to RU entries of row i that lane `sub` of its row group owns (every
 // LPR-th), loaded before any gather; e_tail..e_end is the rest of the lane's
 // entries (rows longer than RU * LPR).  Unused slots: value 0, column 0.
+// Entry range (ptr[i], ptr[i+1]) of row i, (0, 0) past the last row.
+__device__ inline int2 row_ptr(int i, int rows, const int32_t* __restrict__ ptr) {
+    return i < rows ? make_int2(ptr[i], ptr[i + 1]) : make_int2(0, 0);
+}
+
 template <int LPR> struct RowSlot {
     int c[RU];
     float v[RU];
     int e_tail, e_end;
     __device__ inline void load(int i, int rows, int sub, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                 const float* __restrict__ val) {
-        int e0 = 0, e1 = 0;
-        if (i < rows) {
-            e0 = ptr[i] + sub;
-            e1 = ptr[i + 1];
-        }
+        load_range(row_ptr(i, rows, ptr), sub, col, val);
+    }
+    // from a row range loaded beforehand (one dependent load less per row)
+    __device__ inline void load_range(int2 pr, int sub, const int32_t* __restrict__ col, const float* __restrict__ val) {
+        const int e0 = pr.x + sub, e1 = pr.y;
 #pragma unroll
         for (int u = 0; u < RU; u++) {
             const int e = e0 + u * LPR;
@@ -1062,12 +1067,18 @@ template <int LPR> struct RowSlot {
 
 __device__ inline float3 ld3_or0(const float* __restrict__ a, int i, bool ok) { return ok ? ld3(a, i) : f3(0.f, 0.f, 0.f); }
 
+// The fused kernels below take PF = true when their grid makes more than one
+// pass over the rows: each pass then loads the next pass's row ranges while
+// its own gathers are in flight, one dependent load less per pass.  A single
+// pass keeps the variant without the extra registers.
+inline bool multi_pass(int rows, int grid, int lpr) { return (int64_t)rows > 2 * (int64_t)grid * (PBLK / lpr); }
+
 // The three SpMV-type kernels of the fused PCG iteration process two rows per
 // lane group (i and i + one grid's worth of rows) and issue every load that
 // does not depend on the reduced scalars (row pointers, entries, gathers, the
 // row's own values) before reducing the previous kernel's partial sums.
 
-template <int LPR>
+template <int LPR, bool PF>
 __global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot_old, int nparts, double* __restrict__ st,
                                                      const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                      const float* __restrict__ val, const float* __restrict__ diag,
@@ -1079,12 +1090,25 @@ __global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot
     bool need_be = !first;
     double acc[3] = {0, 0, 0};
     auto comb = [&](float3 zc, float3 pc) { return f3(zc.x + be[0] * pc.x, zc.y + be[1] * pc.y, zc.z + be[2] * pc.z); };
-    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
+    const int r_first = xcd_block() * rpb;
+    int2 pra = make_int2(0, 0), prb = make_int2(0, 0);
+    if constexpr (PF) {
+        pra = row_ptr(r_first + threadIdx.x / LPR, n, ptr);
+        prb = row_ptr(r_first + threadIdx.x / LPR + stride, n, ptr);
+    }
+    for (int r0 = r_first; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
-        A.load(ia, n, sub, ptr, col, val);
-        B.load(ib, n, sub, ptr, col, val);
+        if constexpr (PF) {
+            A.load_range(pra, sub, col, val);
+            B.load_range(prb, sub, col, val);
+            pra = row_ptr(ia + 2 * stride, n, ptr); // the next pass's row ranges, in flight meanwhile
+            prb = row_ptr(ib + 2 * stride, n, ptr);
+        } else {
+            A.load(ia, n, sub, ptr, col, val);
+            B.load(ib, n, sub, ptr, col, val);
+        }
         const float3 zia = ld3_or0(z, ia, ha), zib = ld3_or0(z, ib, hb);
         const float3 pia = first ? zia : ld3_or0(p_old, ia, ha), pib = first ? zib : ld3_or0(p_old, ib, hb);
         const float dia = ha ? diag[ia] : 0.f, dib = hb ? diag[ib] : 0.f;
@@ -1141,7 +1165,7 @@ __global__ void __launch_bounds__(PBLK) k_cg_dir_spmv(int n, int first, int slot
     block_sum_write<3, PBLK>(acc, part, 9, 0);
 }
 
-template <int LPR>
+template <int LPR, bool PF>
 __global__ void __launch_bounds__(PBLK) k_cg_update_pre(int n, int slot, int nparts, double* __restrict__ st,
                                                        const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                        const float* __restrict__ val, const float* __restrict__ dinv, float w,
@@ -1167,12 +1191,25 @@ __global__ void __launch_bounds__(PBLK) k_cg_update_pre(int n, int slot, int npa
     };
     auto rnew = [&](float3 rc, float3 qc) { return f3(rc.x - al[0] * qc.x, rc.y - al[1] * qc.y, rc.z - al[2] * qc.z); };
     double acc[3] = {0, 0, 0};
-    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
+    const int r_first = xcd_block() * rpb;
+    int2 pra = make_int2(0, 0), prb = make_int2(0, 0);
+    if constexpr (PF) {
+        pra = row_ptr(r_first + threadIdx.x / LPR, n, ptr);
+        prb = row_ptr(r_first + threadIdx.x / LPR + stride, n, ptr);
+    }
+    for (int r0 = r_first; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
-        A.load(ia, n, sub, ptr, col, val);
-        B.load(ib, n, sub, ptr, col, val);
+        if constexpr (PF) {
+            A.load_range(pra, sub, col, val);
+            B.load_range(prb, sub, col, val);
+            pra = row_ptr(ia + 2 * stride, n, ptr); // the next pass's row ranges, in flight meanwhile
+            prb = row_ptr(ib + 2 * stride, n, ptr);
+        } else {
+            A.load(ia, n, sub, ptr, col, val);
+            B.load(ib, n, sub, ptr, col, val);
+        }
         const float3 ria = ld3_or0(r_old, ia, ha), qia = ld3_or0(q, ia, ha), pia = ld3_or0(pd, ia, ha), dia = ld3_or0(d, ia, ha);
         const float3 rib = ld3_or0(r_old, ib, hb), qib = ld3_or0(q, ib, hb), pib = ld3_or0(pd, ib, hb), dib = ld3_or0(d, ib, hb);
         const float wia = ha ? w * dinv[ia] : 0.f, wib = hb ? w * dinv[ib] : 0.f;
@@ -1268,7 +1305,7 @@ __global__ void __launch_bounds__(BLK) k_amg_spmv(int rows, const int32_t* __res
 // Post-smoothing: xo = xi + w D^-1 (b - A xi); with part != null also b.xo at
 // 6..8 (on the finest level b is the PCG residual, so this is r.z).  Two rows
 // per lane group, as the fused PCG kernels.
-template <int LPR>
+template <int LPR, bool PF>
 __global__ void __launch_bounds__(PBLK) k_amg_post(int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                                                   const float* __restrict__ val, const float* __restrict__ dinv, float w,
                                                   const float* __restrict__ b, const float* __restrict__ xi, float* __restrict__ xo,
@@ -1276,12 +1313,25 @@ __global__ void __launch_bounds__(PBLK) k_amg_post(int n, const int32_t* __restr
     const int sub = threadIdx.x % LPR;
     const int rpb = PBLK / LPR, stride = gridDim.x * rpb;
     double acc[3] = {0, 0, 0};
-    for (int r0 = xcd_block() * rpb; r0 < n; r0 += 2 * stride) {
+    const int r_first = xcd_block() * rpb;
+    int2 pra = make_int2(0, 0), prb = make_int2(0, 0);
+    if constexpr (PF) {
+        pra = row_ptr(r_first + threadIdx.x / LPR, n, ptr);
+        prb = row_ptr(r_first + threadIdx.x / LPR + stride, n, ptr);
+    }
+    for (int r0 = r_first; r0 < n; r0 += 2 * stride) {
         const int ia = r0 + threadIdx.x / LPR, ib = ia + stride;
         const bool ha = ia < n, hb = ib < n;
         RowSlot<LPR> A, B;
-        A.load(ia, n, sub, ptr, col, val);
-        B.load(ib, n, sub, ptr, col, val);
+        if constexpr (PF) {
+            A.load_range(pra, sub, col, val);
+            B.load_range(prb, sub, col, val);
+            pra = row_ptr(ia + 2 * stride, n, ptr); // the next pass's row ranges, in flight meanwhile
+            prb = row_ptr(ib + 2 * stride, n, ptr);
+        } else {
+            A.load(ia, n, sub, ptr, col, val);
+            B.load(ib, n, sub, ptr, col, val);
+        }
         const float3 bia = ld3_or0(b, ia, ha), xia = ld3_or0(xi, ia, ha), bib = ld3_or0(b, ib, hb), xib = ld3_or0(xi, ib, hb);
         const float wia = ha ? w * dinv[ia] : 0.f, wib = hb ? w * dinv[ib] : 0.f;
         float3 xa[RU], xb[RU];
@@ -2100,7 +2150,8 @@ double Backend::spmv_time_ms(int reps) {
             with_lpr(a_lpr, [&](auto c) {
                 constexpr int P = decltype(c)::value;
                 if (fused)
-                    hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, 0, 0, nb, cg_state.p, a_ptr.p, a_col.p,
+                    hipLaunchKernelGGL((multi_pass(n, nb, P) ? k_cg_dir_spmv<P, true> : k_cg_dir_spmv<P, false>), dim3(nb), dim3(PBLK), 0,
+                                       stream, n, 0, 0, nb, cg_state.p, a_ptr.p, a_col.p,
                                        a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)cg_p.p, cg_p2.p, cg_q.p, cg_part.p);
                 else
                     hipLaunchKernelGGL(k_cg_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, a_ptr.p, a_col.p, a_val.p, a_diag.p,
@@ -2187,7 +2238,8 @@ void Backend::precondition(const float* r, bool skip_pre0) {
         with_lpr(L.lpr_a, [&](auto c) {
             constexpr int P = decltype(c)::value;
             const int grid = l == 0 ? nb : grid_for((int64_t)L.n * P, PBLK);
-            hipLaunchKernelGGL(k_amg_post<P>, dim3(grid), dim3(PBLK), 0, stream, L.n, L.aptr.p, L.acol.p, L.aval.p, L.dinv.p, L.omega,
+            hipLaunchKernelGGL((multi_pass(L.n, grid, P) ? k_amg_post<P, true> : k_amg_post<P, false>), dim3(grid), dim3(PBLK), 0, stream,
+                               L.n, L.aptr.p, L.acol.p, L.aval.p, L.dinv.p, L.omega,
                                bl, (const float*)L.x.p, out, l == 0 ? cg_part.p : (double*)nullptr);
         });
         PA_LAUNCH_CHECK();
@@ -2220,7 +2272,8 @@ void Backend::pcg_iteration(int it, bool fused, int& pc, int& rc) {
     if (fused) {
         with_lpr(a_lpr, [&](auto c) {
             constexpr int P = decltype(c)::value;
-            hipLaunchKernelGGL(k_cg_dir_spmv<P>, dim3(nb), dim3(PBLK), 0, stream, n, it == 0 ? 1 : 0, slot ^ 1, nb, cg_state.p, a_ptr.p,
+            hipLaunchKernelGGL((multi_pass(n, nb, P) ? k_cg_dir_spmv<P, true> : k_cg_dir_spmv<P, false>), dim3(nb), dim3(PBLK), 0, stream,
+                               n, it == 0 ? 1 : 0, slot ^ 1, nb, cg_state.p, a_ptr.p,
                                a_col.p, a_val.p, a_diag.p, (const float*)cg_z.p, (const float*)pbuf[pc], pbuf[pc ^ 1], cg_q.p,
                                cg_part.p);
         });
@@ -2241,7 +2294,8 @@ void Backend::pcg_iteration(int it, bool fused, int& pc, int& rc) {
         DevLevel& L0 = *amg[0];
         with_lpr(L0.lpr_a, [&](auto c) {
             constexpr int P = decltype(c)::value;
-            hipLaunchKernelGGL(k_cg_update_pre<P>, dim3(nb), dim3(PBLK), 0, stream, n, slot, nb, cg_state.p, L0.aptr.p, L0.acol.p,
+            hipLaunchKernelGGL((multi_pass(n, nb, P) ? k_cg_update_pre<P, true> : k_cg_update_pre<P, false>), dim3(nb), dim3(PBLK), 0,
+                               stream, n, slot, nb, cg_state.p, L0.aptr.p, L0.acol.p,
                                L0.aval.p, L0.dinv.p, L0.omega, (const float*)pbuf[pc], (const float*)cg_q.p, cg_d.p,
                                (const float*)rbuf[rc], rbuf[rc ^ 1], L0.x.p, L0.r.p, cg_part.p);
         });
