@@ -176,6 +176,33 @@ def test_global_solve_matches_llt(pa, precond):
             assert it <= 60
 
 
+def test_global_solve_multi_pass_matches_llt(pa):
+    """A 300 x 300 cloth (90,000 rows) is more rows than one pass of the fused
+    PCG grid covers, so the kernels run their multi-pass variant (each pass
+    loads the next pass's row ranges ahead): against the oracle's sparse LLT
+    within 1e-6 relative."""
+    from parallel_admm_amd.configs import config_for
+
+    cfg = config_for("batch")
+    cfg.global_tol = 1e-5
+    cfg.global_refine = 2
+    cfg.global_max_iter = 2000
+    sc = pa.Scene.generate(4, side=300, radius=0.2, drop=0.3)
+    m = sc.mesh()
+    n = sc.info["n_dyn"]
+    assert n == 90000
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    o = _oracle(m, cfg)
+    rng = np.random.default_rng(11)
+    xs = m["x"][:n] + 0.01 * rng.standard_normal((n, 3))
+    rhs = o.matrix(1) @ xs
+    xo = o.solve(rhs)
+    xg, it = b.global_solve(rhs, m["x"][:n])
+    err = rel_err(xg, xo)
+    print(f"side 300: PCG {it} iterations, rel err vs LLT {err:.2e}")
+    assert err < 1e-6 and it <= 60
+
+
 STEP_CASES = [
     # (label, config, scene kind, generator args, frames, start state)
     ("cloth32-free", "cloth32", 1, {}, 4, "rest"),
