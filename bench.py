@@ -169,7 +169,7 @@ def learned_leg(pa, args, scene, cfg, tracks, plain_stats, plain_value) -> dict:
     }
 
 
-def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16, graphs: int = 1) -> dict:
+def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16, graphs: int = 0) -> dict:
     """The batched workload: `count` cloths (side ~ U{16..96}, drop ~ U(0.2,
     0.6) m, sphere radius ~ U(0.1, 0.3) m, mu ~ U(0.1, 0.6); generator seed 0;
     h = 1/60, 30 ADMM iterations) merged into one scene and stepped together.
@@ -255,8 +255,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-solo", type=int, default=16, help="scenes of the batch also stepped one at a time")
-    ap.add_argument("--batch-graphs", type=int, default=1, choices=[0, 1],
-                    help="replay the batched PCG iteration pairs as HIP graphs (0: launch every kernel, e.g. under a kernel tracer)")
+    ap.add_argument("--batch-graphs", type=int, default=0, choices=[0, 1],
+                    help="1: replay the batched PCG iteration pairs as HIP graphs (0: launch every kernel, the default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "

@@ -55,6 +55,15 @@ class PAError(RuntimeError):
         self.code = code
 
 
+def _destroy(fn: str, h) -> None:
+    """Frees a handle; at interpreter shutdown the module's globals may already
+    be gone, and the process exit releases the device memory anyway."""
+    try:
+        getattr(lib(), fn)(h)
+    except (TypeError, AttributeError, NameError):
+        pass
+
+
 class AdmmConfigC(C.Structure):
     _fields_ = [
         ("dt", C.c_double), ("g", C.c_double),
@@ -238,7 +247,7 @@ class Scene:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_scene_destroy(self._h)
+            _destroy("pa_scene_destroy", self._h)
             self._h = None
 
     def mesh(self) -> dict:
@@ -311,7 +320,7 @@ class CollisionDetector:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_detector_destroy(self._h)
+            _destroy("pa_detector_destroy", self._h)
             self._h = None
 
     def detect(self, pos) -> int:
@@ -387,7 +396,7 @@ class ADMMBackendHIP:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_backend_destroy(self._h)
+            _destroy("pa_backend_destroy", self._h)
             self._h = None
 
     def _err(self, code, what):
@@ -624,7 +633,7 @@ class MLP:
 
     def __del__(self):
         if getattr(self, "_h", None):
-            lib().pa_mlp_destroy(self._h)
+            _destroy("pa_mlp_destroy", self._h)
             self._h = None
 
     def params(self) -> np.ndarray:

@@ -31,11 +31,9 @@ def config_for(name: str) -> AdmmConfig:
                           kappa=1e6, beta=1e5, thickness=5e-4, damp_kl=1e-4, damp_km=0.0, w_scale=50.0,
                           max_contacts=1 << 21)
     if name == "batch":
-        # the batched step replays its PCG iteration pairs as captured HIP
-        # graphs (bitwise identical to eager launches, same speed)
         return AdmmConfig(dt=1.0 / 60, g=9.81, admm_max_iter=30, dcd_interval=5, gs_max_iter=10, mu=0.3,
                           kappa=1e6, beta=1e5, thickness=1e-3, damp_kl=1e-4, damp_km=0.0, w_scale=50.0,
-                          max_contacts=1 << 16, graphs=1)
+                          max_contacts=1 << 16)
     raise KeyError(name)
 
 

