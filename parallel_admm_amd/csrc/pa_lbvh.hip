@@ -6,7 +6,8 @@
 // -> node boxes from leaf ranges -> 128-byte wide node records (the boxes,
 // ids and last sorted leaf of a node's up to four grandchildren, a leaf child
 // standing in for its own pair, so a traversal step descends two levels with
-// one dependent load) -> one traversal per query with the FP64 predicate,
+// one dependent load; the stack top's record is loaded ahead, so a pop needs
+// none) -> one traversal per query with the FP64 predicate,
 // accepted pairs appended to a staging list with their segment.
 //
 // Point-triangle: one lane per surface vertex; its pairs form the vertex's
@@ -413,11 +414,19 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, 
         leaf(start);
     } else {
         int sp = 0, node = start;
-        while (true) {
-            const float4* rn = rec + 8 * (size_t)node;
-            float4 s[8];
+        float4 s[8], t[8]; // the current node's record; the stack top's, loaded ahead
+        int top = -1;      // the node whose record t holds
 #pragma unroll
-            for (int j = 0; j < 8; j++) s[j] = rn[j];
+        for (int j = 0; j < 8; j++) s[j] = rec[8 * (size_t)node + j];
+        while (true) {
+            // the stack top's record is loaded while this node is processed,
+            // so the next pop costs no dependent load
+            const int st_top = sp > 0 ? stack[sp - 1][threadIdx.x] : -1;
+            if (st_top >= 0 && st_top != top) {
+                top = st_top;
+#pragma unroll
+                for (int j = 0; j < 8; j++) t[j] = rec[8 * (size_t)top + j];
+            }
             int c[4];
             unsigned leaves = 0, inner = 0;
 #pragma unroll
@@ -437,14 +446,24 @@ __global__ void __launch_bounds__(QBLK) k_query_pt(const float* __restrict__ x, 
                 inner &= inner - 1;
                 node = pick4(j, c[0], c[1], c[2], c[3]);
                 while (inner) {
-                    const int t = __ffs(inner) - 1;
+                    const int u = __ffs(inner) - 1;
                     inner &= inner - 1;
-                    if (sp < STACK) stack[sp++][threadIdx.x] = pick4(t, c[0], c[1], c[2], c[3]);
+                    if (sp < STACK) stack[sp++][threadIdx.x] = pick4(u, c[0], c[1], c[2], c[3]);
                     else atomicOr(&flags[1], 1u);
                 }
+#pragma unroll
+                for (int q2 = 0; q2 < 8; q2++) s[q2] = rec[8 * (size_t)node + q2];
             } else {
                 if (sp == 0) break;
                 node = stack[--sp][threadIdx.x];
+                if (node == top) {
+#pragma unroll
+                    for (int q2 = 0; q2 < 8; q2++) s[q2] = t[q2];
+                    top = -1;
+                } else {
+#pragma unroll
+                    for (int q2 = 0; q2 < 8; q2++) s[q2] = rec[8 * (size_t)node + q2];
+                }
             }
         }
     }
@@ -494,11 +513,19 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, 
     const int start = roots ? roots[vseg[a0]] : 0;
     if (start < 0 || start >= leaf0) return;
     int sp = 0, node = start;
-    while (true) {
-        const float4* rn = rec + 8 * (size_t)node;
-        float4 s[8];
+    float4 s[8], t[8]; // the current node's record; the stack top's, loaded ahead
+    int top = -1;      // the node whose record t holds
 #pragma unroll
-        for (int j = 0; j < 8; j++) s[j] = rn[j];
+    for (int j = 0; j < 8; j++) s[j] = rec[8 * (size_t)node + j];
+    while (true) {
+        // The next pop takes the stack top: its record is loaded while this
+        // node is processed, so a pop costs no dependent load.
+        const int st_top = sp > 0 ? stack[sp - 1][threadIdx.x] : -1;
+        if (st_top >= 0 && st_top != top) {
+            top = st_top;
+#pragma unroll
+            for (int j = 0; j < 8; j++) t[j] = rec[8 * (size_t)top + j];
+        }
         int c[4];
         unsigned leaves = 0, inner = 0;
 #pragma unroll
@@ -518,14 +545,24 @@ __global__ void __launch_bounds__(QBLK) k_query_ee(const float* __restrict__ x, 
             inner &= inner - 1;
             node = pick4(j, c[0], c[1], c[2], c[3]);
             while (inner) {
-                const int t = __ffs(inner) - 1;
+                const int u = __ffs(inner) - 1;
                 inner &= inner - 1;
-                if (sp < STACK) stack[sp++][threadIdx.x] = pick4(t, c[0], c[1], c[2], c[3]);
+                if (sp < STACK) stack[sp++][threadIdx.x] = pick4(u, c[0], c[1], c[2], c[3]);
                 else atomicOr(&flags[1], 1u);
             }
+#pragma unroll
+            for (int q = 0; q < 8; q++) s[q] = rec[8 * (size_t)node + q];
         } else {
             if (sp == 0) break;
             node = stack[--sp][threadIdx.x];
+            if (node == top) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) s[q] = t[q];
+                top = -1;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) s[q] = rec[8 * (size_t)node + q];
+            }
         }
     }
 }
