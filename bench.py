@@ -169,6 +169,37 @@ def learned_leg(pa, args, scene, cfg, tracks, plain_stats, plain_value) -> dict:
     }
 
 
+def full_run_leg(pa, workload: str, scene, frames: int) -> dict:
+    """The workload's whole run from its initial state (cloth256: 300 frames,
+    fall, draping and folding over the sphere and the rotating cylinder) on a
+    fresh backend, with the same settings as the timed steps: the rate over
+    all frames, the contact peak and the ADMM iterations early and late."""
+    from parallel_admm_amd.configs import config_for, workload_tracks
+
+    cfg = config_for(workload)
+    cfg.residual_tol = 1e-4
+    b = pa.ADMMBackendHIP.from_scene(scene, cfg)
+    b.set_sync_to_host(False)
+    for first, count, pos, quat in workload_tracks(workload, scene, frames + 8):
+        b.add_rigid_track(first, count, pos, quat)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        stats.append(b.step())
+    wall = time.perf_counter() - t0
+    x, _ = b.get_state()
+    it = np.array([s["iterations"] for s in stats], float)
+    q = max(1, frames // 10)
+    out = {"frames": frames, "steps_per_s": frames / wall, "seconds": wall,
+           "contacts_max": int(max(s["n_contacts"] for s in stats)), "contacts_last": int(stats[-1]["n_contacts"]),
+           "admm_iters_first_tenth": float(it[:q].mean()), "admm_iters_last_tenth": float(it[-q:].mean()),
+           "converged_steps": int(sum(s["converged"] for s in stats)), "state_finite": bool(np.isfinite(x).all())}
+    print(f"[bench] {workload} whole run: {frames} frames in {wall:.1f} s = {out['steps_per_s']:.1f} steps/s, contacts max "
+          f"{out['contacts_max']}, ADMM iterations {out['admm_iters_first_tenth']:.1f} -> {out['admm_iters_last_tenth']:.1f}",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def batch_leg(pa, count: int, warmup: int, steps: int, solo_sample: int = 16, graphs: int = 0) -> dict:
     """The batched workload: `count` cloths (side ~ U{16..96}, drop ~ U(0.2,
     0.6) m, sphere radius ~ U(0.1, 0.3) m, mu ~ U(0.1, 0.6); generator seed 0;
@@ -255,6 +286,8 @@ def main() -> int:
     ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--batch-warmup", type=int, default=20)
     ap.add_argument("--batch-solo", type=int, default=16, help="scenes of the batch also stepped one at a time")
+    ap.add_argument("--full-run-frames", type=int, default=300,
+                    help="cloth256: also run the workload's whole run (300 frames) on a fresh backend and report its rate (0 skips it)")
     ap.add_argument("--batch-graphs", type=int, default=0, choices=[0, 1],
                     help="1: replay the batched PCG iteration pairs as HIP graphs (0: launch every kernel, the default)")
     args = ap.parse_args()
@@ -332,6 +365,10 @@ def main() -> int:
 
     learned = None if args.no_learned else learned_leg(pa, args, scene, cfg, tracks, stats, args.steps / elapsed)
 
+    full = None
+    if args.full_run_frames > 0 and args.workload == "cloth256":
+        full = full_run_leg(pa, args.workload, scene, args.full_run_frames)
+
     batch = None
     if args.batch_scenes > 0:
         try:
@@ -389,6 +426,7 @@ def main() -> int:
         },
         "cpu_baseline": cpu,
         "learned": learned,
+        "full_run": full,
         "batch": batch,
     }
     print(json.dumps(out), flush=True)
