@@ -346,6 +346,42 @@ def test_step_overflow_rolls_back(pa):
     np.testing.assert_array_equal(b.get_state()[0], before)
 
 
+def test_cloth256_folded_step_matches_fp64_oracle(pa):
+    """cloth256 at the workload's contact density: 120 steps of the whole run
+    (sphere and rotating cylinder) into a folded state with over ten thousand
+    contacts, then one step from that common state on both, the oracle's
+    cylinder track resumed at the same frame: within 1e-5 relative, contact
+    counts of the step's last detection within 0.5% (see
+    test_cloth256_step_matches_fp64_oracle for why they may differ by a few)."""
+    from oracle.pyoracle import Oracle
+    from parallel_admm_amd.configs import config_for, workload_tracks
+
+    cfg = config_for("cloth256")
+    sc = pa.Scene.generate(3)
+    warm = 120
+    tracks = workload_tracks("cloth256", sc, warm + 8)
+    b = pa.ADMMBackendHIP.from_scene(sc, cfg)
+    b.set_sync_to_host(False)
+    for first, count, pos, quat in tracks:
+        b.add_rigid_track(first, count, pos, quat)
+    for _ in range(warm):
+        b.step()
+    o = Oracle(sc.mesh(), cfg)
+    for first, count, pos, quat in tracks:
+        o.add_rigid_track(first, count, pos[warm:], quat[warm:])
+    x, v = b.get_state()
+    ue, uc = b.get_duals()
+    o.set_state(x, v)
+    o.set_duals(ue, uc)
+    st = b.step()
+    so = o.step()
+    e = rel_err(b.get_state()[0], o.state()[0])
+    print(f"cloth256 frame {warm}: contacts gpu {st['n_contacts']} oracle {so['n_contacts']}, ADMM iterations {st['iterations']}, "
+          f"PCG {st['global_iters']}, rel err {e:.2e}")
+    assert st["n_contacts"] > 5000 and abs(st["n_contacts"] - so["n_contacts"]) <= 0.005 * so["n_contacts"]
+    assert e < 1e-5
+
+
 def test_nonfinite_state_rolls_back(pa):
     """NaN injected into x (or Inf into v) from the host: the next step
     reports PA_ERR_NUMERICAL and the state is the one after the last good
