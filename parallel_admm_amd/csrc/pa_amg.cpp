@@ -212,6 +212,7 @@ AmgHierarchy amg_build(const HostCsr& A0, double theta, int coarse_max, int max_
                 if (A.col[e] == i) L.dinv[i] = 1.0 / A.val[e];
         const double rho = spectral_radius_dinv_a(A, L.dinv);
         L.omega = 4.0 / (3.0 * rho);
+        L.omega_smooth = 5.0 / (3.0 * rho);
         const bool last = A.rows <= coarse_max || (int)H.levels.size() + 1 >= max_levels;
         if (!last) {
             // Dense coarse operators make the strength graph dense and the

@@ -83,7 +83,7 @@ struct Backend {
     // AMG preconditioner (device copy of the hierarchy built at init)
     struct DevLevel {
         int n = 0, nnext = 0;
-        float omega = 0.f;
+        float omega = 0.f; // weight of the V-cycle's Jacobi sweeps (AmgLevel::omega_smooth)
         int lpr_a = 2, lpr_p = 2, lpr_r = 2; // lanes per row for A, P and R
         DBuf<int32_t> aptr, acol, pptr, pcol, rptr, rcol;
         DBuf<float> aval, dinv, pval, rval;

@@ -1847,7 +1847,7 @@ void Backend::assemble_and_upload(const double* x0h) {
                          "AMG level with column indices out of range");
                 auto D = std::make_unique<DevLevel>();
                 D->n = L.A.rows;
-                D->omega = (float)L.omega;
+                D->omega = (float)L.omega_smooth;
                 D->lpr_a = lanes_per_row(L.A.nnz(), L.A.rows);
                 amg_sizes.push_back(D->n);
                 std::vector<float> av(L.A.val.begin(), L.A.val.end()), dv(L.dinv.begin(), L.dinv.end());
